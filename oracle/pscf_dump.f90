@@ -1,0 +1,243 @@
+!-----------------------------------------------------------------------
+! pscf_dump -- test infrastructure: golden-vector dump driver.
+!
+! Links the reference PSCF modules (built by oracle/Makefile into
+! oracle/_ref) and writes the intermediate arrays of the SCFT inner loop as
+! raw little-endian binary files, so that the oracle restatement and the
+! HIP path can be pinned against the reference's own arithmetic.
+!
+! Input (stdin) follows the reference's parameter-file format
+! (src/pscf.fp.f90:207-386 for MONOMERS..BASIS), followed by dump commands:
+!
+!   DUMP_TABLES   prefix            star tables, half-grid wave map, ksq,
+!                                   cell bases, dGsq      (a12-a14, a9 input)
+!   DUMP_DENSITY  omega_file prefix density(), rho_grid, omega_grid, Q,
+!                                   scf_stress, mu, f, p  (a7-a10)
+!   DUMP_STEP     q_file monomer ds prefix
+!                                   make_propg + one step_exp (a3/a4)
+!   DUMP_FFT      r_file prefix     fft (r2c) and ifft of the result (a5)
+!   FINISH
+!
+! Every array is written in Fortran (column-major) order, one file per
+! array: <prefix>.<name>.f64 | .c128 | .i32.
+!-----------------------------------------------------------------------
+program pscf_dump
+   use const_mod
+   use io_mod
+   use version_mod
+   use chemistry_mod
+   use unit_cell_mod
+   use grid_mod
+   use basis_mod
+   use fft_mod
+   use grid_basis_mod
+   use step_mod
+   use scf_mod
+   implicit none
+
+   type(version_type)        :: version
+   character(60)             :: op_string
+   character(256)            :: prefix, fname
+   character(60)             :: group_name
+   real(long)                :: chain_step, ds, f_helm, pressure
+   integer                   :: i, j, k, n, mon, i1, i2, i3, idx
+   integer                   :: kbz(3), kfft(3)
+   real(long), allocatable   :: omega(:,:), rho(:,:), q(:), q_sol(:)
+   real(long), allocatable   :: dGsq(:,:), stress(:), rgrid(:,:,:), rout(:,:,:)
+   real(long), allocatable   :: dbuf(:)
+   complex(long), allocatable:: kgrid(:,:,:)
+   integer, allocatable      :: ibuf(:)
+   logical                   :: started = .false.
+
+   call set_echo(0)
+   call set_com_style('A','A','A')
+   call set_com_use('R')
+   call set_io_units(i=5,o=6)
+   call input_version(version, 5)
+
+   op_loop: do
+      read(5,*)
+      read(5,*) op_string
+      select case (trim(op_string))
+      case ('MONOMERS')
+         call input_monomers(5,'F')
+      case ('CHAINS')
+         call input_chains(5,'F')
+      case ('SOLVENTS')
+         call input_solvents(5,'F')
+      case ('COMPOSITION')
+         call input_composition(5,'F')
+      case ('INTERACTION')
+         call input_interaction(5,'F')
+      case ('UNIT_CELL')
+         call input_unit_cell(5,'F')
+         call make_unit_cell
+      case ('DISCRETIZATION')
+         call input_grid
+         call allocate_grid(N_monomer)
+         call input(chain_step,'chain_step')
+      case ('BASIS')
+         call input(group_name,'group_name')
+         call make_basis(R_basis,G_basis,group_name,ngrid,grid_flag=.TRUE.)
+         allocate(omega(N_monomer,N_star), rho(N_monomer,N_star))
+         allocate(q(max(N_chain,1)), q_sol(max(N_solvent,1)))
+         allocate(dGsq(N_star,N_cell_param), stress(N_cell_param))
+         allocate(rgrid(0:ngrid(1)-1,0:ngrid(2)-1,0:ngrid(3)-1))
+         allocate(rout(0:ngrid(1)-1,0:ngrid(2)-1,0:ngrid(3)-1))
+         allocate(kgrid(0:ngrid(1)/2,0:ngrid(2)-1,0:ngrid(3)-1))
+         write(6,*) 'N_star', N_star, 'N_wave', N_wave
+
+      case ('DUMP_TABLES')
+         call input(prefix,'prefix')
+         call make_ksq(G_basis)
+         allocate(ibuf(4))
+         ibuf(1) = dim; ibuf(2) = N_wave; ibuf(3) = N_star; ibuf(4) = N_cell_param
+         call wr_i(trim(prefix)//'.sizes', ibuf, 4)
+         deallocate(ibuf)
+         call wr_i(trim(prefix)//'.ngrid', ngrid, 3)
+         call wr_i(trim(prefix)//'.wave', wave, dim*N_wave)
+         call wr_c(trim(prefix)//'.coeff', coeff, N_wave)
+         call wr_i(trim(prefix)//'.star_of_wave', star_of_wave, N_wave)
+         call wr_i(trim(prefix)//'.star_begin', star_begin, N_star)
+         call wr_i(trim(prefix)//'.star_end', star_end, N_star)
+         call wr_i(trim(prefix)//'.star_count', star_count, N_star)
+         call wr_i(trim(prefix)//'.star_invert', star_invert, N_star)
+         call wr_i(trim(prefix)//'.wave_of_star', wave_of_star, dim*N_star)
+         call wr_d(trim(prefix)//'.R_basis', R_basis, dim*dim)
+         call wr_d(trim(prefix)//'.G_basis', G_basis, dim*dim)
+         call wr_d(trim(prefix)//'.dGG_basis', dGG_basis, size(dGG_basis))
+         call wr_d(trim(prefix)//'.cell_param', cell_param, 6)
+         call wr_d(trim(prefix)//'.ksq', ksq_grid, size(ksq_grid))
+         ! half-grid map used by basis_to_kgrid: k -> (G_to_bz(k), which_wave)
+         n = (ngrid(1)/2+1)*ngrid(2)*ngrid(3)
+         allocate(ibuf(4*n))
+         idx = 0
+         do i3 = 0, ngrid(3)-1
+            do i2 = 0, ngrid(2)-1
+               do i1 = 0, ngrid(1)/2
+                  kfft = (/ i1, i2, i3 /)
+                  kbz = G_to_bz(kfft)
+                  ibuf(idx+1:idx+3) = kbz
+                  ibuf(idx+4) = which_wave(kbz(1),kbz(2),kbz(3))
+                  idx = idx + 4
+               end do
+            end do
+         end do
+         call wr_i(trim(prefix)//'.kmap', ibuf, 4*n)
+         deallocate(ibuf)
+         do i = 1, N_cell_param
+            call make_dGsq(dGsq(:,i), dGG_basis(:,:,i))
+         end do
+         call wr_d(trim(prefix)//'.dGsq', dGsq, N_star*N_cell_param)
+
+      case ('DUMP_DENSITY')
+         call input(fname,'omega_file')
+         call input(prefix,'prefix')
+         call startup()
+         call rd_d(trim(fname), omega, N_monomer*N_star)
+         call density(N_star, omega, rho, q, q_sol)
+         call wr_d(trim(prefix)//'.rho', rho, N_monomer*N_star)
+         call wr_d(trim(prefix)//'.Q', q, N_chain)
+         call wr_d(trim(prefix)//'.omega_grid', omega_grid, size(omega_grid))
+         call wr_d(trim(prefix)//'.rho_grid', rho_grid, size(rho_grid))
+         do i = 1, N_cell_param
+            call make_dGsq(dGsq(:,i), dGG_basis(:,:,i))
+         end do
+         stress = scf_stress(N_star, N_cell_param, dGsq)
+         call wr_d(trim(prefix)//'.stress', stress, N_cell_param)
+         call mu_phi_chain(mu_chain, phi_chain, q)
+         if (N_solvent > 0) then
+            call mu_phi_solvent(mu_solvent, phi_solvent, q_sol)
+            call wr_d(trim(prefix)//'.Q_solvent', q_sol, N_solvent)
+            call wr_d(trim(prefix)//'.mu_solvent', mu_solvent, N_solvent)
+         end if
+         call free_energy(N_star, rho, omega, phi_chain, mu_chain, &
+                          phi_solvent, mu_solvent, f_helm, pressure)
+         allocate(dbuf(2+N_chain))
+         dbuf(1) = f_helm; dbuf(2) = pressure; dbuf(3:2+N_chain) = mu_chain(1:N_chain)
+         call wr_d(trim(prefix)//'.thermo', dbuf, 2+N_chain)
+         deallocate(dbuf)
+         ! grid<->basis maps on the omega field of monomer 1
+         call basis_to_kgrid(omega(1,:), kgrid)
+         call wr_c(trim(prefix)//'.kgrid_omega1', kgrid, size(kgrid))
+
+      case ('DUMP_STEP')
+         call input(fname,'q_file')
+         call input(mon,'monomer')
+         call input(ds,'ds')
+         call input(prefix,'prefix')
+         call startup()
+         call rd_d(trim(fname), rgrid, size(rgrid))
+         call make_propg(ds, kuhn(mon), ksq_grid, omega_grid(:,:,:,mon))
+         call step_exp(rgrid, rout, plan)
+         call wr_d(trim(prefix)//'.q_out', rout, size(rout))
+
+      case ('DUMP_FFT')
+         call input(fname,'r_file')
+         call input(prefix,'prefix')
+         call startup()
+         call rd_d(trim(fname), rgrid, size(rgrid))
+         call fft(plan, rgrid, kgrid)
+         call wr_c(trim(prefix)//'.fft', kgrid, size(kgrid))
+         allocate(dbuf(N_star))
+         call kgrid_to_basis(kgrid, dbuf)
+         call wr_d(trim(prefix)//'.k2b', dbuf, N_star)
+         deallocate(dbuf)
+         call ifft(plan, kgrid, rout)
+         call wr_d(trim(prefix)//'.ifft', rout, size(rout))
+
+      case ('FINISH')
+         exit op_loop
+      case default
+         write(6,*) 'pscf_dump: unknown op ', trim(op_string)
+         stop 1
+      end select
+   end do op_loop
+
+contains
+
+   subroutine startup()
+      if (.not. started) then
+         call make_ksq(G_basis)
+         call density_startup(ngrid, 1, chain_step, update_chain=.false.)
+         started = .true.
+      end if
+   end subroutine startup
+
+   subroutine wr_d(name, a, n)
+      character(*), intent(IN) :: name
+      integer, intent(IN)      :: n
+      real(long), intent(IN)   :: a(*)
+      open(31, file=name//'.f64', access='stream', form='unformatted', status='replace')
+      write(31) a(1:n)
+      close(31)
+   end subroutine wr_d
+
+   subroutine wr_c(name, a, n)
+      character(*), intent(IN)   :: name
+      integer, intent(IN)        :: n
+      complex(long), intent(IN)  :: a(*)
+      open(31, file=name//'.c128', access='stream', form='unformatted', status='replace')
+      write(31) a(1:n)
+      close(31)
+   end subroutine wr_c
+
+   subroutine wr_i(name, a, n)
+      character(*), intent(IN) :: name
+      integer, intent(IN)      :: n
+      integer, intent(IN)      :: a(*)
+      open(31, file=name//'.i32', access='stream', form='unformatted', status='replace')
+      write(31) a(1:n)
+      close(31)
+   end subroutine wr_i
+
+   subroutine rd_d(name, a, n)
+      character(*), intent(IN)  :: name
+      integer, intent(IN)       :: n
+      real(long), intent(OUT)   :: a(*)
+      open(32, file=name, access='stream', form='unformatted', status='old')
+      read(32) a(1:n)
+      close(32)
+   end subroutine rd_d
+
+end program pscf_dump

@@ -1,0 +1,114 @@
+"""System description for one SCFT instance, in the reference's vocabulary.
+
+Mirrors the globals the reference reads from its parameter script:
+chemistry_mod (src/chemistry_mod.f90:48-84: kuhn, chains of blocks,
+phi_chain, solvents, ensemble, chi), unit_cell_mod (src/unit_cell_mod.f90:35-43:
+dim, crystal_system, cell_param), grid_mod (src/grid_mod.f90:43: ngrid) and
+the DISCRETIZATION chain_step (src/pscf.fp.f90:374-379).  ``to_param`` writes
+the same sections in the reference's parameter-file format
+(src/pscf.fp.f90:207-412), so one System can drive both this library and the
+reference program.
+"""
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+
+def _fmt(x):
+    return "%20.10E" % float(x)
+
+
+@dataclass
+class System:
+    kuhn: Sequence[float]
+    chains: Sequence[Sequence[Tuple[int, float]]]  # per chain: (monomer 1-based, length)
+    phi_chain: Sequence[float]
+    chi: np.ndarray
+    dim: int
+    crystal_system: str
+    cell_param: Sequence[float]
+    ngrid: Sequence[int]
+    chain_step: float
+    group_name: str
+    ensemble: int = 0
+    mu_chain: Sequence[float] = None
+    solvents: Sequence[Tuple[int, float]] = field(default_factory=list)  # (monomer, size)
+    phi_solvent: Sequence[float] = field(default_factory=list)
+
+    def __post_init__(self):
+        self.chi = np.asarray(self.chi, dtype=float)
+        if self.mu_chain is None:
+            self.mu_chain = [0.0] * len(self.chains)
+
+    @property
+    def N_monomer(self):
+        return len(self.kuhn)
+
+    @property
+    def N_chain(self):
+        return len(self.chains)
+
+    @property
+    def ngrid3(self):
+        g = list(self.ngrid) + [1] * (3 - len(self.ngrid))
+        return [int(v) for v in g[:3]]
+
+    def chain_length(self, i):
+        s = 0.0
+        for _, L in self.chains[i]:
+            s = s + L
+        return s
+
+    def phi_monomer(self):
+        """Average monomer volume fractions (src/scf_mod.fp.f90:734-744)."""
+        phi = np.zeros(self.N_monomer)
+        for i, ch in enumerate(self.chains):
+            L = self.chain_length(i)
+            for mon, bl in ch:
+                phi[mon - 1] = phi[mon - 1] + self.phi_chain[i] * bl / L
+        for (mon, _), p in zip(self.solvents, self.phi_solvent):
+            phi[mon - 1] = phi[mon - 1] + p
+        return phi
+
+    def to_param(self) -> str:
+        """MONOMERS .. BASIS sections in the reference's format."""
+        L = ["format  1  0", "", "MONOMERS", "N_monomer", "%20d" % self.N_monomer,
+             "kuhn", "".join(_fmt(k) for k in self.kuhn), "", "CHAINS", "N_chain",
+             "%20d" % self.N_chain, "N_block"]
+        L += ["%20d" % len(c) for c in self.chains]
+        L += ["block_monomer"] + ["".join("%20d" % m for m, _ in c) for c in self.chains]
+        L += ["block_length"] + ["".join(_fmt(b) for _, b in c) for c in self.chains]
+        L += [""]
+        if self.solvents:
+            L += ["SOLVENTS", "N_solvent", "%20d" % len(self.solvents), "solvent_monomer",
+                  "".join("%20d" % m for m, _ in self.solvents), "solvent_size",
+                  "".join(_fmt(s) for _, s in self.solvents), ""]
+        L += ["COMPOSITION", "ensemble", "%20d" % self.ensemble]
+        if self.ensemble == 0:
+            L += ["phi_chain"] + [_fmt(p) for p in self.phi_chain]
+            if self.solvents:
+                L += ["phi_solvent"] + [_fmt(p) for p in self.phi_solvent]
+        else:
+            L += ["mu_chain"] + [_fmt(p) for p in self.mu_chain]
+        L += ["", "INTERACTION", "interaction_type", "               'chi'", "chi"]
+        for i in range(1, self.N_monomer):
+            L.append("".join(_fmt(self.chi[i, j]) for j in range(i)))
+        L += ["", "UNIT_CELL", "dim", "%20d" % self.dim, "crystal_system",
+              "%20s" % ("'" + self.crystal_system + "'"), "N_cell_param",
+              "%20d" % len(self.cell_param), "cell_param",
+              "".join(_fmt(c) for c in self.cell_param), "", "DISCRETIZATION", "ngrid",
+              "".join("%20d" % n for n in self.ngrid[: self.dim]), "chain_step",
+              _fmt(self.chain_step), "", "BASIS", "group_name",
+              "%20s" % ("'" + self.group_name + "'"), ""]
+        return "\n".join(L) + "\n"
+
+
+def diblock(fA, N, chi_N, *, dim, crystal_system, cell_param, ngrid, chain_step,
+            group_name, bA=1.0, bB=1.0):
+    """AB diblock melt with block lengths fA*N and (1-fA)*N."""
+    chi = chi_N / N
+    return System(kuhn=[bA, bB], chains=[[(1, fA * N), (2, (1.0 - fA) * N)]],
+                  phi_chain=[1.0], chi=np.array([[0.0, chi], [chi, 0.0]]), dim=dim,
+                  crystal_system=crystal_system, cell_param=list(cell_param),
+                  ngrid=list(ngrid), chain_step=chain_step, group_name=group_name)

@@ -1,0 +1,174 @@
+"""Write the golden fixtures under tests/golden/ from the reference build.
+
+Runs the reference PSCF modules through the dump driver (oracle/_ref/pscf_dump,
+built by ``make -C oracle`` from the reference sources) on seeded synthetic
+inputs, and stores inputs and outputs as compressed .npz fixtures.  Only this
+script touches the reference build; tests read the fixtures.
+
+    python tests/golden/make_golden.py [case ...]
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from pscf_amd.system import System, diblock  # noqa: E402
+
+DUMP = os.path.join(ROOT, "oracle", "_ref", "pscf_dump")
+
+
+def _chi2(c):
+    return np.array([[0.0, c], [c, 0.0]])
+
+
+def cases():
+    """name -> (System, seed, options)."""
+    C = {}
+    # BCC 8^3, small diblock: every array dumped
+    C["bcc8"] = (diblock(0.3, 10.0, 20.0, dim=3, crystal_system="cubic", cell_param=[5.0],
+                         ngrid=[8, 8, 8], chain_step=0.5, group_name="I m -3 m"),
+                 20261290, dict(grids=True, nstar_seed=6))
+    # lamellar, 1D, 32 points, the known-answer fixture's parameters
+    C["lam32"] = (System(kuhn=[1.0, 1.0], chains=[[(1, 0.5), (2, 0.5)]], phi_chain=[1.0],
+                         chi=_chi2(15.000000011), dim=1, crystal_system="lamellar",
+                         cell_param=[1.5161093081], ngrid=[32], chain_step=0.01,
+                         group_name="-1"),
+                  20261291, dict(grids=True, nstar_seed=6))
+    # 2D hexagonal, radix-3 grid
+    C["hex24"] = (System(kuhn=[1.0, 1.2], chains=[[(1, 0.3), (2, 0.7)]], phi_chain=[1.0],
+                         chi=_chi2(20.0), dim=2, crystal_system="hexagonal",
+                         cell_param=[1.8], ngrid=[24, 24], chain_step=0.02,
+                         group_name="p 6 m m"),
+                  20261292, dict(grids=True, nstar_seed=8))
+    # non-centrosymmetric group: star_invert = +-1 pairs
+    C["p213_12"] = (System(kuhn=[1.0, 1.0], chains=[[(1, 0.4), (2, 0.6)]], phi_chain=[1.0],
+                           chi=_chi2(18.0), dim=3, crystal_system="cubic", cell_param=[2.0],
+                           ngrid=[12, 12, 12], chain_step=0.05, group_name="P 21 3"),
+                    20261293, dict(grids=True, nstar_seed=12))
+    # ABA triblock + A homopolymer blend + B solvent, BCC 12^3 (3 blocks, 2 chains)
+    C["blend12"] = (System(kuhn=[1.0, 1.0], chains=[[(1, 0.2), (2, 0.6), (1, 0.2)], [(1, 0.5)]],
+                           phi_chain=[0.8, 0.15], chi=_chi2(20.0), dim=3,
+                           crystal_system="cubic", cell_param=[2.2], ngrid=[12, 12, 12],
+                           chain_step=0.05, group_name="I m -3 m",
+                           solvents=[(2, 0.1)], phi_solvent=[0.05]),
+                    20261294, dict(grids=True, nstar_seed=8))
+    # BCC 32^3: the N=100, 25/75, chi N=20, a=18, ds=1 system
+    C["bcc32"] = (diblock(0.25, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[18.0],
+                          ngrid=[32, 32, 32], chain_step=1.0, group_name="I m -3 m"),
+                  20261200, dict(grids=False, step=True, nstar_seed=11))
+    # gyroid 64^3: the N=100, 36/64, chi N=20, a=16, ds=0.5 system
+    C["gyr64"] = (diblock(0.36, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[16.0],
+                          ngrid=[64, 64, 64], chain_step=0.5, group_name="I a -3 d"),
+                  20261201, dict(grids=False, step=False, nstar_seed=21))
+    return C
+
+
+def seeded_omega(sysm, N_star, seed, nstar_seed):
+    """Star 1 = chi . phi_mon (the set_omega_uniform convention,
+    src/scf_mod.fp.f90:724-750); stars 2..nstar_seed ~ N(0, 0.5*chi*N/N)
+    per monomer, i.e. sigma = 0.5*chi; the rest 0."""
+    rng = np.random.default_rng(seed)
+    chi = float(sysm.chi[0, 1])
+    om = np.zeros((sysm.N_monomer, N_star))
+    phi = sysm.phi_monomer()
+    om[:, 0] = sysm.chi @ phi
+    n = min(nstar_seed, N_star)
+    om[:, 1:n] = rng.normal(0.0, 0.5 * chi, size=(sysm.N_monomer, n - 1))
+    return om
+
+
+def run_case(name, sysm, seed, opt, outdir):
+    with tempfile.TemporaryDirectory() as td:
+        p = lambda s: os.path.join(td, s)
+        # pass 1: tables
+        script = sysm.to_param() + "DUMP_TABLES\nprefix\n'%s'\n\nFINISH\n" % p("t")
+        subprocess.run([DUMP], input=script.encode(), check=True, cwd=td,
+                       stdout=subprocess.DEVNULL)
+        rd = lambda s, dt: np.fromfile(p(s), dtype=dt)
+        dim, N_wave, N_star, ncp = rd("t.sizes.i32", np.int32)
+        ng = rd("t.ngrid.i32", np.int32)
+        N1, N2, N3 = [int(v) for v in ng]
+        d = dict(dim=dim, ngrid=ng, N_cell_param=ncp,
+                 wave=rd("t.wave.i32", np.int32).reshape(N_wave, dim),
+                 coeff=rd("t.coeff.c128", np.complex128),
+                 star_of_wave=rd("t.star_of_wave.i32", np.int32),
+                 star_begin=rd("t.star_begin.i32", np.int32),
+                 star_end=rd("t.star_end.i32", np.int32),
+                 star_count=rd("t.star_count.i32", np.int32),
+                 star_invert=rd("t.star_invert.i32", np.int32),
+                 wave_of_star=rd("t.wave_of_star.i32", np.int32).reshape(N_star, dim),
+                 R_basis=rd("t.R_basis.f64", np.float64).reshape(dim, dim, order="F"),
+                 G_basis=rd("t.G_basis.f64", np.float64).reshape(dim, dim, order="F"),
+                 dGG_basis=rd("t.dGG_basis.f64", np.float64).reshape(dim, dim, 6, order="F"),
+                 cell_param=rd("t.cell_param.f64", np.float64),
+                 ksq=rd("t.ksq.f64", np.float64).reshape(N3, N2, N1 // 2 + 1),
+                 kmap=rd("t.kmap.i32", np.int32).reshape(N3, N2, N1 // 2 + 1, 4),
+                 dGsq=rd("t.dGsq.f64", np.float64).reshape(N_star, ncp, order="F"))
+        # inputs
+        omega = seeded_omega(sysm, int(N_star), seed, opt["nstar_seed"])
+        np.asfortranarray(omega).T.tofile(p("omega.bin"))  # Fortran (N_monomer, N_star)
+        rng = np.random.default_rng(seed + 1000)
+        q_in = 1.0 + 0.3 * rng.random((N3, N2, N1))
+        r_in = rng.normal(size=(N3, N2, N1))
+        q_in.tofile(p("q.bin"))
+        r_in.tofile(p("r.bin"))
+        d["omega"] = omega
+        ops = "DUMP_DENSITY\nomega_file\n'%s'\nprefix\n'%s'\n\n" % (p("omega.bin"), p("d"))
+        want_step = opt.get("step", True)
+        ops_step = ""
+        if want_step:
+            ds = float(sysm.chain_step)
+            d["step_ds"] = ds
+            d["step_monomer"] = sysm.chains[0][0][0]
+            d["q_in"] = q_in
+            ops_step = "DUMP_STEP\nq_file\n'%s'\nmonomer\n%d\nds\n%.17E\nprefix\n'%s'\n\n" % (
+                p("q.bin"), sysm.chains[0][0][0], ds, p("s"))
+        ops_fft = ""
+        if opt["grids"]:
+            d["r_in"] = r_in
+            ops_fft = "DUMP_FFT\nr_file\n'%s'\nprefix\n'%s'\n\n" % (p("r.bin"), p("f"))
+        script = sysm.to_param() + ops + ops_step + ops_fft + "FINISH\n"
+        subprocess.run([DUMP], input=script.encode(), check=True, cwd=td,
+                       stdout=subprocess.DEVNULL)
+        nm = sysm.N_monomer
+        d["rho"] = rd("d.rho.f64", np.float64).reshape(nm, N_star, order="F")
+        d["Q"] = rd("d.Q.f64", np.float64)
+        d["stress"] = rd("d.stress.f64", np.float64)
+        d["thermo"] = rd("d.thermo.f64", np.float64)
+        if sysm.solvents:
+            d["Q_solvent"] = rd("d.Q_solvent.f64", np.float64)
+            d["mu_solvent"] = rd("d.mu_solvent.f64", np.float64)
+        if opt["grids"]:
+            d["omega_grid"] = rd("d.omega_grid.f64", np.float64).reshape(nm, N3, N2, N1)
+            d["rho_grid"] = rd("d.rho_grid.f64", np.float64).reshape(nm, N3, N2, N1)
+            d["kgrid_omega1"] = rd("d.kgrid_omega1.c128", np.complex128).reshape(
+                N3, N2, N1 // 2 + 1)
+            d["fft_out"] = rd("f.fft.c128", np.complex128).reshape(N3, N2, N1 // 2 + 1)
+            d["k2b"] = rd("f.k2b.f64", np.float64)
+            d["ifft_out"] = rd("f.ifft.f64", np.float64).reshape(N3, N2, N1)
+        if want_step:
+            d["step_q_out"] = rd("s.q_out.f64", np.float64).reshape(N3, N2, N1)
+        d["param"] = np.frombuffer(sysm.to_param().encode(), dtype=np.uint8)
+        out = os.path.join(outdir, name + ".npz")
+        np.savez_compressed(out, **d)
+        return out
+
+
+def main(argv):
+    if not os.path.exists(DUMP):
+        raise SystemExit("build the reference dump driver first: make -C oracle")
+    C = cases()
+    names = argv or list(C)
+    for n in names:
+        sysm, seed, opt = C[n]
+        out = run_case(n, sysm, seed, opt, HERE)
+        print(n, out, os.path.getsize(out))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,283 @@
+"""Pin the oracle (oracle/pscf_oracle.py) before it is trusted as the checker.
+
+Each oracle function is checked against (i) the reference's own arithmetic,
+dumped by oracle/_ref/pscf_dump into tests/golden/*.npz, and (ii) what the
+mathematics fixes independently of any code: closed forms, brute-force DFTs,
+invariants and finite differences.  These run on the CPU only.
+"""
+import itertools
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_golden
+
+import pscf_oracle as O
+
+sys.path.insert(0, os.path.join(GOLDEN))
+from make_golden import cases  # noqa: E402
+
+SMALL = ["bcc8", "lam32", "hex24", "p213_12", "blend12"]
+
+
+def chem_of(sysm):
+    return O.Chemistry(sysm.kuhn, sysm.chains, sysm.phi_chain, sysm.chi, sysm.ensemble,
+                       solvents=sysm.solvents, phi_solvent=sysm.phi_solvent)
+
+
+def oracle_system(name, recompute_kmap=True):
+    g = load_golden(name)
+    sysm = cases()[name][0]
+    T = O.Tables(g, G_basis=g["G_basis"] if recompute_kmap else None)
+    ksq = O.make_ksq(T.ngrid, g["G_basis"])
+    scf = O.SCF(chem_of(sysm), T, ksq, sysm.chain_step)
+    return g, sysm, T, ksq, scf
+
+
+# --------------------------------------------------------------------------
+# a1: contour discretisation
+# --------------------------------------------------------------------------
+
+def test_chain_grid_hand_worked():
+    # 25/75 at ds=1: iblk=int(12.5+0.5)=13, int(37.5+0.5)=38 -> 26+76 = 102 steps
+    bgn, bds = O.chain_grid([25.0, 75.0], 1.0)
+    assert bgn == [1, 27, 103]
+    assert bds == [25.0 / 13 / 2, 75.0 / 38 / 2]
+    # 36/64 at ds=0.5 -> 72+128 = 200 steps, block_ds exactly 0.5
+    bgn, bds = O.chain_grid([36.0, 64.0], 0.5)
+    assert bgn == [1, 73, 201] and bds == [0.5, 0.5]
+    # ABA 20/60/20 at ds=0.5 -> 40+120+40
+    bgn, _ = O.chain_grid([20.0, 60.0, 20.0], 0.5)
+    assert bgn == [1, 41, 161, 201]
+    # a block shorter than ds/2 rounds to iblk=0 -> one double step, ds=L/2
+    bgn, bds = O.chain_grid([0.2, 1.0], 1.0)
+    assert bgn[1] == 3 and bds[0] == 0.1
+    # order halves ds and doubles the count
+    bgn, bds = O.chain_grid([25.0, 75.0], 1.0, order=1)
+    assert bgn == [1, 53, 205] and bds[0] == 25.0 / 13 / 2 / 2
+
+
+# --------------------------------------------------------------------------
+# a14: G_to_bz / ksq -- bit-exact against the reference
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", SMALL + ["bcc32"])
+def test_kmap_and_ksq_match_reference(name):
+    g = load_golden(name)
+    T = O.Tables(g, G_basis=g["G_basis"])
+    assert np.array_equal(T.kmap, g["kmap"].astype(np.int64))
+    ksq = O.make_ksq(T.ngrid, g["G_basis"])
+    np.testing.assert_allclose(ksq, g["ksq"], rtol=2e-16, atol=0)
+
+
+def test_g_to_bz_tie_rule_brute_force():
+    # square 2D cell: G=(N/2, *) ties between +N/2 and -N/2; the rule keeps the
+    # first trial in order +N, 0, -N that is strictly smaller by 1e-8.
+    Gb = np.eye(2) * 2 * np.pi
+    for G in [(4, 0), (4, 4), (0, 4), (3, 5), (7, 7)]:
+        got = O.G_to_bz(list(G), [8, 8, 1], Gb)
+        cands = []
+        for t in itertools.product((1, 0, -1), repeat=2):
+            Gt = [G[0] + 8 * t[0], G[1] + 8 * t[1]]
+            cands.append((Gt[0] ** 2 + Gt[1] ** 2, Gt))
+        best = min(c[0] for c in cands)
+        first = [c[1] for c in cands if c[0] == best][0]
+        assert got[:2] == first
+
+
+# --------------------------------------------------------------------------
+# a5: FFT conventions
+# --------------------------------------------------------------------------
+
+def test_fft_brute_force_dft():
+    rng = np.random.default_rng(1)
+    N1, N2, N3 = 4, 3, 2
+    r = rng.normal(size=(N3, N2, N1))
+    k = O.fft(r)
+    for (k3, k2, k1) in itertools.product(range(N3), range(N2), range(N1 // 2 + 1)):
+        s = 0.0
+        for (x3, x2, x1) in itertools.product(range(N3), range(N2), range(N1)):
+            s += r[x3, x2, x1] * np.exp(-2j * np.pi * (k1 * x1 / N1 + k2 * x2 / N2 + k3 * x3 / N3))
+        assert abs(s - k[k3, k2, k1]) < 1e-12
+    # unnormalised round trip: ifft(fft(r)) = M r
+    np.testing.assert_allclose(O.ifft(k, r.shape), r.size * r, rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_fft_matches_reference(name):
+    g = load_golden(name)
+    k = O.fft(g["r_in"])
+    scale = np.abs(g["fft_out"]).max()
+    np.testing.assert_allclose(k, g["fft_out"], rtol=0, atol=1e-13 * scale)
+    r = O.ifft(g["fft_out"], g["r_in"].shape)
+    np.testing.assert_allclose(r, g["ifft_out"], rtol=0, atol=1e-13 * np.abs(r).max())
+
+
+# --------------------------------------------------------------------------
+# a12/a13: star basis <-> k grid
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", SMALL)
+def test_basis_to_kgrid_matches_reference(name):
+    g = load_golden(name)
+    T = O.Tables(g, G_basis=g["G_basis"])
+    kg = O.basis_to_kgrid(g["omega"][0], T)
+    # zero pattern (which k points carry a wave) is index work: exact
+    np.testing.assert_array_equal(kg == 0, g["kgrid_omega1"] == 0)
+    # values: the complex product (a -+ ib) c / sqrt(2) may round differently by 1 ulp
+    np.testing.assert_allclose(kg, g["kgrid_omega1"], rtol=0,
+                               atol=4e-16 * np.abs(g["kgrid_omega1"]).max())
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_kgrid_to_basis_matches_reference(name):
+    g = load_golden(name)
+    T = O.Tables(g, G_basis=g["G_basis"])
+    # z/c: complex division may round differently by 1 ulp
+    np.testing.assert_allclose(O.kgrid_to_basis(g["fft_out"], T), g["k2b"], rtol=0,
+                               atol=4e-16 * np.abs(g["k2b"]).max())
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_basis_roundtrip_identity(name):
+    # kgrid_to_basis(basis_to_kgrid(x)) = x for every star, incl. +-1 pairs
+    g = load_golden(name)
+    T = O.Tables(g, G_basis=g["G_basis"])
+    x = np.random.default_rng(3).normal(size=T.N_star)
+    np.testing.assert_allclose(O.kgrid_to_basis(O.basis_to_kgrid(x, T), T), x,
+                               rtol=1e-14, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# a3/a4: propagator step
+# --------------------------------------------------------------------------
+
+def test_step_uniform_field_closed_form():
+    ng = [6, 4, 5]
+    Gb = np.diag([1.1, 0.9, 1.3])
+    ksq = O.make_ksq(ng, Gb)
+    P = O.Propagator(ng)
+    w, ds = 0.37, 0.21
+    P.make_propg(ds, 1.3, ksq, np.full((5, 4, 6), w))
+    q = np.full((5, 4, 6), 2.0)
+    np.testing.assert_allclose(P.step_exp(q), 2.0 * np.exp(-w * ds), rtol=1e-14)
+
+
+def test_step_plane_wave_closed_form():
+    ng = [8, 8, 8]
+    Gb = np.eye(3) * 2 * np.pi / 3.0
+    ksq = O.make_ksq(ng, Gb)
+    P = O.Propagator(ng)
+    ds, b = 0.3, 1.2
+    P.make_propg(ds, b, ksq, np.zeros((8, 8, 8)))
+    x = np.arange(8)
+    G = (1, 2, 0)
+    q = np.cos(2 * np.pi * (G[0] * x[None, None, :] + G[1] * x[None, :, None]) / 8)
+    q = np.broadcast_to(q, (8, 8, 8)).copy()
+    gsq = ((2 * np.pi / 3.0) ** 2) * (G[0] ** 2 + G[1] ** 2)
+    np.testing.assert_allclose(P.step_exp(q), np.exp(-ds * b * b * gsq / 6.0) * q,
+                               rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", SMALL + ["bcc32"])
+def test_step_matches_reference(name):
+    g, sysm, T, ksq, scf = oracle_system(name)
+    if "step_q_out" not in g:
+        pytest.skip("no step dump")
+    P = O.Propagator(T.ngrid)
+    mon = int(g["step_monomer"])
+    if "omega_grid" in g:
+        wg = g["omega_grid"][mon - 1]
+    else:
+        wg = O.ifft(O.basis_to_kgrid(g["omega"][mon - 1], T), g["q_in"].shape)
+    P.make_propg(float(g["step_ds"]), sysm.kuhn[mon - 1], ksq, wg)
+    out = P.step_exp(g["q_in"])
+    np.testing.assert_allclose(out, g["step_q_out"], rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# a7-a10: density, Q, stress, thermodynamics
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", SMALL + ["bcc32"])
+def test_density_matches_reference(name):
+    g, sysm, T, ksq, scf = oracle_system(name)
+    rho, q = scf.density(g["omega"])
+    np.testing.assert_allclose(q, g["Q"], rtol=1e-11)
+    if "omega_grid" in g:
+        np.testing.assert_allclose(scf.omega_grid, g["omega_grid"], rtol=0,
+                                   atol=1e-13 * np.abs(g["omega_grid"]).max())
+        np.testing.assert_allclose(scf.rho_grid, g["rho_grid"], rtol=1e-10)
+    np.testing.assert_allclose(rho, g["rho"], rtol=0, atol=1e-11 * np.abs(g["rho"]).max())
+    if sysm.solvents:
+        np.testing.assert_allclose(scf.q_solvent, g["Q_solvent"], rtol=1e-13)
+    stress = scf.scf_stress(g["dGsq"])
+    np.testing.assert_allclose(stress, g["stress"], rtol=1e-10)
+    O.mu_phi_chain(scf.chem, q)
+    if sysm.solvents:
+        O.mu_phi_solvent(scf.chem, scf.q_solvent)
+    f, p = O.free_energy(scf.chem, rho, g["omega"])
+    np.testing.assert_allclose([f, p], g["thermo"][:2], rtol=1e-10)
+    np.testing.assert_allclose(scf.chem.mu_chain, g["thermo"][2:2 + sysm.N_chain], rtol=1e-12)
+
+
+def test_make_dgsq_matches_reference():
+    for name in SMALL:
+        g = load_golden(name)
+        T = O.Tables(g)
+        for i in range(int(g["N_cell_param"])):
+            np.testing.assert_array_equal(O.make_dGsq(T, g["dGG_basis"][:, :, i]), g["dGsq"][:, i])
+
+
+def test_homogeneous_closed_forms():
+    # uniform omega per monomer: Q = exp(-sum_b w_b L_b); rho_alpha = phi L_alpha / N;
+    # stress = 0; F equals the Flory-Huggins free energy when omega = chi . phi.
+    g, sysm, T, ksq, scf = oracle_system("blend12")
+    om = np.zeros_like(g["omega"])
+    om[:, 0] = [0.7, -0.4]
+    rho, q = scf.density(om)
+    for i, ch in enumerate(sysm.chains):
+        expect = np.exp(-sum(om[m - 1, 0] * L for m, L in ch))
+        assert abs(q[i] / expect - 1) < 1e-13
+    # star 1 of rho = total volume fraction per monomer (solvent included)
+    O.mu_phi_chain(scf.chem, q)
+    O.mu_phi_solvent(scf.chem, scf.q_solvent)
+    np.testing.assert_allclose(rho[:, 0], sysm.phi_monomer(), rtol=1e-13)
+    np.testing.assert_allclose(rho[:, 1:], 0.0, atol=1e-13)
+    np.testing.assert_allclose(scf.scf_stress(g["dGsq"]), 0.0, atol=1e-13)
+    om = O.set_omega_uniform(scf.chem, np.zeros_like(om))
+    rho, q = scf.density(om)
+    O.mu_phi_chain(scf.chem, q)
+    O.mu_phi_solvent(scf.chem, scf.q_solvent)
+    f, _ = O.free_energy(scf.chem, rho, om)
+    assert abs(f - O.free_energy_FH(scf.chem)) < 1e-12
+
+
+def test_stress_equals_fd_of_lnQ():
+    # scf_stress = -(phi/N) dlnQ/da at fixed omega stars; for a cubic cell
+    # ksq scales as (a0/a)^2.  The perturbation formula and the derivative of
+    # the discrete Q differ by contour-discretisation error only, so the gap
+    # must shrink at least quadratically as ds is refined (a wrong factor or
+    # sign would not).
+    g = load_golden("bcc8")
+    sysm = cases()["bcc8"][0]
+    T = O.Tables(g)
+    ksq0 = O.make_ksq(T.ngrid, g["G_basis"])
+    om = g["omega"].copy()
+    om[:, 1:] *= 0.3
+    a0 = float(g["cell_param"][0])
+    h = 1e-4 * a0
+    errs = []
+    for ds in (0.5, 0.125):
+        scf = O.SCF(chem_of(sysm), T, ksq0, ds)
+        scf.density(om)
+        stress = scf.scf_stress(g["dGsq"])[0]
+        lq = []
+        for a in (a0 + h, a0 - h):
+            scf.ksq = ksq0 * (a0 / a) ** 2
+            lq.append(np.log(scf.density(om)[1][0]))
+        fd = -sysm.phi_chain[0] / sysm.chain_length(0) * (lq[0] - lq[1]) / (2 * h)
+        errs.append(abs(stress - fd) / abs(fd))
+    assert errs[0] < 2e-2 and errs[1] < errs[0] / 12, errs
