@@ -1,0 +1,194 @@
+/*
+ * pscf_hip.h -- C ABI of the MI355X (gfx950) SCFT inner loop.
+ *
+ * Drop-in boundary for the reference PSCF Fortran modules fft_mod, step_mod,
+ * chain_mod, scf_mod and grid_basis_mod.  The replacement Fortran modules in
+ * pscf_amd/fortran/ keep the reference module interfaces and forward to these
+ * entry points through ISO_C_BINDING; Python binds them with ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero status on failure;
+ *     pscf_last_error() then describes the failure.  (The Fortran shims turn
+ *     a nonzero status into `stop`, the reference's error convention.)
+ *   - Arrays are FP64 and laid out as the reference's Fortran arrays: grids
+ *     (0:N1-1,0:N2-1,0:N3-1) with x fastest; half spectra (0:N1/2,0:N2-1,0:N3-1)
+ *     as interleaved (re,im) pairs; star arrays omega(N_monomer,N_star).
+ *   - Functions without the _dev suffix take host pointers and return with
+ *     their outputs final (the device stream is synchronised).  _dev functions
+ *     take device pointers and are asynchronous on the handle's stream.
+ *   - Handles are opaque 64-bit integers (they fit the integer*8 fields of
+ *     the reference's fft_plan type).
+ *   - Transforms are unnormalised, as in the reference (callers divide by M).
+ */
+#ifndef PSCF_HIP_H
+#define PSCF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pscf_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int pscf_device_count(int* count);
+
+/* ---------------------------------------------------------------- fft_mod
+ * Replaces src/fft3_mod.f90.  ngrid = (N1,N2,N3), unused dims 1
+ * (src/grid_mod.f90:89-91). */
+
+/* create_fft_plan(ngrid, plan, fft_c2c)          src/fft3_mod.f90:61-67 */
+int pscf_fft_plan_create(const int ngrid[3], int64_t* plan);
+int pscf_fft_plan_destroy(int64_t plan);
+/* fft(plan, in, out): real -> half spectrum, sign -1  src/fft3_mod.f90:85-94 */
+int pscf_fft_r2c(int64_t plan, const double* in, double* out);
+/* ifft(plan, in, out): half spectrum -> real, sign +1  src/fft3_mod.f90:113-122 */
+int pscf_fft_c2r(int64_t plan, const double* in, double* out);
+/* fftc(direction, plan, in, out): full complex grid; direction +1 uses sign
+ * -1 and divides by M, -1 uses sign +1            src/fft3_mod.f90:140-157 */
+int pscf_fft_c2c(int64_t plan, int direction, const double* in, double* out);
+/* device-pointer forms of fft / ifft (nbatch arrays back to back) */
+int pscf_fft_r2c_dev(int64_t plan, const double* in, double* out, int nbatch);
+int pscf_fft_c2r_dev(int64_t plan, const double* in, double* out, int nbatch);
+
+/* --------------------------------------------------------------- step_mod
+ * Replaces src/step_mod.f90 (module state held by the handle). */
+
+/* init_step(n)                                    src/step_mod.f90:60-92 */
+int pscf_step_create(const int ngrid[3], int64_t* step);
+int pscf_step_destroy(int64_t step);
+/* make_propg(ds, b, ksq, omega): ksq on the half grid, omega on the grid
+ *                                                 src/step_mod.f90:109-129 */
+int pscf_step_make_propg(int64_t step, double ds, double b, const double* ksq, const double* omega);
+/* step_exp(q_in, q_out, plan)                     src/step_mod.f90:146-176 */
+int pscf_step_exp(int64_t step, const double* q_in, double* q_out);
+
+/* -------------------------------------------------------------- chain_mod
+ * make_chain_grid's contour discretisation   src/chain_mod.f90:97-175.
+ * order < 0 means the optional argument is absent.  block_bgn (nblk+1) is
+ * 1-based as in the reference. */
+int pscf_chain_discretize(int nblk, const double* blk_length, double ds0, int order,
+                          int* block_bgn, double* block_ds);
+
+/* ---------------------------------------------------------------- scf_mod
+ * One SCFT system (the reference keeps it in module globals:
+ * src/scf_mod.fp.f90:53-55, src/grid_mod.f90:43-46). */
+
+int pscf_system_create(const int ngrid[3], int dim, int64_t* sys);
+int pscf_system_destroy(int64_t sys);
+
+/* Star-basis tables from basis_mod (src/basis_mod.f90:60-84): wave(dim,N_wave),
+ * coeff(N_wave) complex, star_of_wave(N_wave), star_invert(N_star),
+ * wave_of_star(dim,N_star); indices 1-based as in the reference. */
+int pscf_system_set_basis(int64_t sys, int n_wave, int n_star, const int* wave, const double* coeff,
+                          const int* star_of_wave, const int* star_invert, const int* wave_of_star);
+
+/* Cell geometry: G_basis(dim,dim) (src/unit_cell_mod.f90:42).  Recomputes
+ * ksq (make_ksq, src/grid_mod.f90:141-160) and the k -> wave map of
+ * basis_to_kgrid (G_to_bz, src/grid_mod.f90:225-283) on the device. */
+int pscf_system_set_cell(int64_t sys, const double* G_basis);
+/* Copy the device ksq (half grid) to the host. */
+int pscf_system_get_ksq(int64_t sys, double* ksq);
+
+/* Chemistry (src/chemistry_mod.f90:48-84): kuhn(N_monomer);
+ * N_block(N_chain); block_monomer(N_blk_max,N_chain) and
+ * block_length(N_blk_max,N_chain) (Fortran order, 1-based monomers);
+ * solvent_monomer/solvent_size(N_solvent); ensemble 0 canonical, 1 grand. */
+int pscf_system_set_chemistry(int64_t sys, int n_monomer, const double* kuhn, int n_chain,
+                              const int* n_block, int n_blk_max, const int* block_monomer,
+                              const double* block_length, int n_solvent,
+                              const int* solvent_monomer, const double* solvent_size,
+                              int ensemble);
+
+/* Flory-Huggins chi(N_monomer, N_monomer) (src/chemistry_mod.f90:68-70). */
+int pscf_system_set_chi(int64_t sys, const double* chi);
+
+/* density_startup(N_grids, extr_order, chain_step, update_chain)
+ * src/scf_mod.fp.f90:82-117: discretises the chains and (re)allocates the
+ * contour slabs; call again after set_chemistry changes block lengths. */
+int pscf_system_startup(int64_t sys, double chain_step);
+
+/* density(N, omega, rho, qout, q_solvent)        src/scf_mod.fp.f90:144-229
+ * omega, rho: (N_monomer, N_star).  phi_chain/mu_chain(N_chain) and
+ * phi_solvent/mu_solvent(N_solvent) are the chemistry_mod globals: read
+ * (canonical) or updated via mu_phi_* (grand ensemble, :202-209).
+ * qout(N_chain), q_solvent(N_solvent) may be NULL. */
+int pscf_density(int64_t sys, const double* omega, double* rho, double* qout, double* q_solvent,
+                 double* phi_chain, double* mu_chain, double* phi_solvent, double* mu_solvent);
+
+/* scf_stress(N, size_dGsq, dGsq)                 src/scf_mod.fp.f90:451-550
+ * dGsq(N_star, ncp); uses the slabs of the last density().  stress(ncp). */
+int pscf_scf_stress(int64_t sys, int ncp, const double* dGsq, const double* phi_chain,
+                    const double* mu_chain, double* stress);
+
+/* mu_phi_chain(mu, phi, q)                      src/scf_mod.fp.f90:768-785 */
+int pscf_mu_phi_chain(int64_t sys, double* mu, double* phi, const double* q);
+/* mu_phi_solvent(mu, phi, q)                    src/scf_mod.fp.f90:803-820 */
+int pscf_mu_phi_solvent(int64_t sys, double* mu, double* phi, const double* q);
+/* free_energy(N, rho, omega, phi_chain, mu_chain, phi_solvent, mu_solvent,
+ *             f_Helmholtz, pressure)            src/scf_mod.fp.f90:834-886
+ * pressure may be NULL. */
+int pscf_free_energy(int64_t sys, const double* rho, const double* omega, const double* phi_chain,
+                     const double* mu_chain, const double* phi_solvent, const double* mu_solvent,
+                     double* f_helmholtz, double* pressure);
+/* free_energy_FH(phi_chain, phi_solvent)         src/scf_mod.fp.f90:903-941
+ * phi_solvent may be NULL (absent). */
+int pscf_free_energy_fh(int64_t sys, const double* phi_chain, const double* phi_solvent, double* f);
+/* set_omega_uniform(omega): omega(:,1) = chi . phi_mon  src/scf_mod.fp.f90:724-750 */
+int pscf_set_omega_uniform(int64_t sys, double* omega, const double* phi_chain,
+                           const double* phi_solvent);
+
+/* divide_energy head/tail sums (DEVEL variant)   src/scf_mod.fp.f90:588-709:
+ * out(2*N_chain) = (fhead_i, ftail_i) = sum_r [q>0] qf qr log(q) / (Q M) at
+ * block_bgn(2) with log(qf), and at block_bgn(N_block) with log(qr). */
+int pscf_divide_energy_terms(int64_t sys, double* out);
+
+/* Copy the grid fields of the last density(): omega_grid / rho_grid
+ * (N1,N2,N3,N_monomer) (src/grid_mod.f90:44-45); either may be NULL. */
+int pscf_system_get_grids(int64_t sys, double* omega_grid, double* rho_grid);
+
+/* Copy contour point s (0-based) of chain c's qf (dir=+1) or qr (dir=-1). */
+int pscf_system_get_q(int64_t sys, int chain, int dir, int s, double* q);
+
+/* ---------------------------------------------------------- grid_basis_mod
+ * basis_to_kgrid(basis, kgrid, karray_full, no_parity)
+ *                                          src/grid_basis_mod.f90:47-129
+ * kgrid: half spectrum ((N1/2+1)*N2*N3 complex). */
+int pscf_basis_to_kgrid(int64_t sys, const double* basis, double* kgrid, int no_parity);
+/* kgrid_to_basis(kgrid, basis, no_parity)  src/grid_basis_mod.f90:146-200 */
+int pscf_kgrid_to_basis(int64_t sys, const double* kgrid, double* basis, int no_parity);
+
+/* ----------------------------------------------- iteration support (host)
+ * These run on the CPU in FP64, as in the reference; their failures are
+ * reported by pscf_am_last_error(). */
+const char* pscf_am_last_error(void);
+
+/* make_unit_cell for crystal_system/cell_param   src/unit_cell_mod.f90:309-664
+ * (3D cubic, tetragonal, orthorhombic, hexagonal; 2D square, rectangular,
+ * hexagonal; 1D lamellar).  Outputs R_basis(dim,dim), G_basis(dim,dim),
+ * dGG_basis(dim,dim,6); any may be NULL. */
+int pscf_unit_cell(int dim, const char* crystal_system, int n_cell_param, const double* cell_param,
+                   double* R_basis, double* G_basis, double* dGG_basis);
+/* make_dGsq(dGsq, dGG)                      src/basis_mod.f90:1993-2014 */
+int pscf_make_dgsq(int dim, int n_star, const int* wave_of_star, const double* dGG, double* dGsq);
+
+/* Anderson mixing of iterate_AM               src/iterate_mod.fp.f90:983-1490.
+ * create: iterate_AM_startup (:983-1021).  start: history after
+ * set_omega_uniform (:1135-1141).  iterate: one loop pass after density()
+ * (+ scf_stress when domain): residuals, convergence test, mixing update of
+ * omega(:,2:) and cell_param; status 0 updated, 1 converged (error <
+ * error_max), 2 itr > max_itr, 3 diverged. */
+int pscf_am_create(int n_star, int n_monomer, int n_cell_param, int n_hist, int domain,
+                   int64_t* am);
+int pscf_am_destroy(int64_t am);
+int pscf_am_start(int64_t am, const double* omega, const double* cell_param);
+int pscf_am_iterate(int64_t am, double* omega, const double* rho, const double* chi,
+                    const double* stress, double* cell_param, double error_max, int max_itr,
+                    double* error, int* status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSCF_HIP_H */

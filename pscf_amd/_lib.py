@@ -1,0 +1,129 @@
+"""ctypes binding of the C ABI in include/pscf_hip.h.
+
+The product path loads ``pscf_amd/libpscf_hip.so`` (built for gfx950 by
+``pscf_amd/build.py``) and raises if it is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_LIB = os.path.join(HERE, "libpscf_hip.so")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_i64_p = ctypes.POINTER(ctypes.c_int64)
+
+# name -> argtypes (every function returns int status)
+SIGNATURES = {
+    "pscf_device_count": [_c_int_p],
+    "pscf_fft_plan_create": [_c_int_p, _c_i64_p],
+    "pscf_fft_plan_destroy": [ctypes.c_int64],
+    "pscf_fft_r2c": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
+    "pscf_fft_c2r": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
+    "pscf_fft_c2c": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p],
+    "pscf_fft_r2c_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "pscf_fft_c2r_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "pscf_step_create": [_c_int_p, _c_i64_p],
+    "pscf_step_destroy": [ctypes.c_int64],
+    "pscf_step_make_propg": [ctypes.c_int64, ctypes.c_double, ctypes.c_double, _c_dbl_p, _c_dbl_p],
+    "pscf_step_exp": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
+    "pscf_chain_discretize": [ctypes.c_int, _c_dbl_p, ctypes.c_double, ctypes.c_int, _c_int_p,
+                              _c_dbl_p],
+    "pscf_system_create": [_c_int_p, ctypes.c_int, _c_i64_p],
+    "pscf_system_destroy": [ctypes.c_int64],
+    "pscf_system_set_basis": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p,
+                              _c_int_p, _c_int_p, _c_int_p],
+    "pscf_system_set_cell": [ctypes.c_int64, _c_dbl_p],
+    "pscf_system_get_ksq": [ctypes.c_int64, _c_dbl_p],
+    "pscf_system_set_chemistry": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, ctypes.c_int, _c_int_p,
+                                  ctypes.c_int, _c_int_p, _c_dbl_p, ctypes.c_int, _c_int_p,
+                                  _c_dbl_p, ctypes.c_int],
+    "pscf_system_set_chi": [ctypes.c_int64, _c_dbl_p],
+    "pscf_system_startup": [ctypes.c_int64, ctypes.c_double],
+    "pscf_mu_phi_chain": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_mu_phi_solvent": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_free_energy": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                         _c_dbl_p, _c_dbl_p],
+    "pscf_free_energy_fh": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_set_omega_uniform": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_density": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                     _c_dbl_p, _c_dbl_p],
+    "pscf_scf_stress": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_divide_energy_terms": [ctypes.c_int64, _c_dbl_p],
+    "pscf_system_get_grids": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
+    "pscf_system_get_q": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p],
+    "pscf_basis_to_kgrid": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
+    "pscf_kgrid_to_basis": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
+    "pscf_unit_cell": [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                       _c_dbl_p],
+    "pscf_make_dgsq": [ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p],
+    "pscf_am_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                       _c_i64_p],
+    "pscf_am_destroy": [ctypes.c_int64],
+    "pscf_am_start": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
+    "pscf_am_iterate": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                        ctypes.c_double, ctypes.c_int, _c_dbl_p, _c_int_p],
+}
+
+# host-side iteration helpers report through their own error slot
+AM_FUNCS = {"pscf_unit_cell", "pscf_make_dgsq", "pscf_am_create", "pscf_am_destroy",
+            "pscf_am_start", "pscf_am_iterate"}
+
+
+class PscfError(RuntimeError):
+    pass
+
+
+class Lib:
+    """A loaded libpscf_hip (or a library exporting the same C ABI)."""
+
+    def __init__(self, path=None):
+        path = path or DEFAULT_LIB
+        if not os.path.exists(path):
+            raise PscfError("libpscf_hip.so not built at %s: run `python -c 'import "
+                            "__graft_entry__ as g; g.build()'`" % path)
+        self.path = path
+        self.dll = ctypes.CDLL(path)
+        self.dll.pscf_last_error.restype = ctypes.c_char_p
+        self.dll.pscf_last_error.argtypes = []
+        self.dll.pscf_am_last_error.restype = ctypes.c_char_p
+        self.dll.pscf_am_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            fn = getattr(self.dll, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+
+    def call(self, name, *args):
+        rc = getattr(self.dll, name)(*args)
+        if rc != 0:
+            err = self.dll.pscf_am_last_error if name in AM_FUNCS else self.dll.pscf_last_error
+            raise PscfError("%s failed: %s" % (name, err().decode()))
+
+    def device_count(self):
+        n = ctypes.c_int(0)
+        self.call("pscf_device_count", ctypes.byref(n))
+        return n.value
+
+
+_default = None
+
+
+def lib():
+    """The product library (gfx950 build), loaded once."""
+    global _default
+    if _default is None:
+        _default = Lib(DEFAULT_LIB)
+    return _default
+
+
+def dptr(a):
+    """double* of a C-contiguous float64/complex128 numpy array."""
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.float64, np.complex128), (a.dtype, a.flags)
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def iptr(a):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.int32
+    return a.ctypes.data_as(_c_int_p)

@@ -1,0 +1,63 @@
+"""Build libpscf_hip.so for gfx950 in-tree (pscf_amd/libpscf_hip.so).
+
+    python -m pscf_amd.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+SOURCES = ["pscf_hip.hip", "am.cpp"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed-FP32 instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build_lib(verbose=False):
+    srcs = [os.path.join(HERE, "csrc", f) for f in SOURCES]
+    out = os.path.join(HERE, "libpscf_hip.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + srcs + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libpscf_hip.so")
+    os.replace(tmp, out)
+    return out
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+# dependency order of the drop-in modules
+FORTRAN_MODULES = ["pscf_hip_c", "fft_mod", "chain_mod", "step_mod", "grid_basis_mod", "scf_mod"]
+
+
+def build_fortran(verbose=False):
+    """Compile the Fortran drop-in modules (pscf_amd/fortran) to objects and
+    .mod files under pscf_amd/fortran/_build.  They `use` the reference's own
+    kept modules (const_mod, grid_mod, basis_mod, chemistry_mod, ...), whose
+    .mod files a driver build provides with -I; here only pscf_hip_c (the
+    ISO_C_BINDING interface block) is compiled standalone, as a syntax check."""
+    out = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(out, exist_ok=True)
+    src = os.path.join(FORTRAN_DIR, "pscf_hip_c.f90")
+    if not os.path.exists(src):
+        return None
+    cmd = [FLANG, "-O2", "-c", src, "-J", out, "-o", os.path.join(out, "pscf_hip_c.o")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))
+    print(build_fortran(verbose=True))

@@ -1,0 +1,399 @@
+// Host-side iteration support (FP64, CPU): unit-cell geometry, dGsq and the
+// Anderson-mixing update of the reference's iterate_AM.  These are O(N_star)
+// per iteration and stay on the host, as in the reference.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pscf_hip.h"
+
+namespace {
+
+thread_local std::string g_am_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_am_err = e.what();
+    return 1;
+  }
+}
+
+// ------------------------------------------------------------- unit cell
+// src/unit_cell_mod.f90:309-664 (make_unit_cell) and :723-756 (make_G_basis).
+// R(i,j) is component j of lattice vector i; dR(i,j,k) its derivative with
+// respect to cell parameter k.
+void inverse3(const double a[3][3], double out[3][3], int dim) {
+  double m[3][6];
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < 2 * dim; ++j) m[i][j] = j < dim ? a[i][j] : (j - dim == i ? 1.0 : 0.0);
+  for (int c = 0; c < dim; ++c) {
+    int p = c;
+    for (int r = c + 1; r < dim; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+    if (m[p][c] == 0.0) throw std::runtime_error("singular R_basis");
+    if (p != c)
+      for (int j = 0; j < 2 * dim; ++j) std::swap(m[p][j], m[c][j]);
+    const double d = m[c][c];
+    for (int j = 0; j < 2 * dim; ++j) m[c][j] /= d;
+    for (int r = 0; r < dim; ++r)
+      if (r != c) {
+        const double f = m[r][c];
+        for (int j = 0; j < 2 * dim; ++j) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) out[i][j] = m[i][j + dim];
+}
+
+void unit_cell(int dim, const std::string& cs, int ncp, const double* cp, double R[3][3],
+               double G[3][3], double dGG[3][3][6]) {
+  double dR[3][3][6];
+  std::memset(R, 0, sizeof(double) * 9);
+  std::memset(dR, 0, sizeof dR);
+  std::memset(dGG, 0, sizeof(double) * 54);
+  auto need = [&](int n) {
+    if (ncp != n) throw std::runtime_error("Incorrect N_cell_param");
+  };
+  if (dim == 3) {
+    if (cs == "cubic") {
+      need(1);
+      const double a = cp[0];
+      R[0][0] = a; R[1][1] = a; R[2][2] = a;
+      dR[0][0][0] = 1.0; dR[1][1][0] = 1.0; dR[2][2][0] = 1.0;
+    } else if (cs == "tetragonal") {
+      need(2);
+      R[0][0] = cp[0]; R[1][1] = cp[0]; R[2][2] = cp[1];
+      dR[0][0][0] = 1.0; dR[1][1][0] = 1.0; dR[2][2][1] = 1.0;
+    } else if (cs == "orthorhombic") {
+      need(3);
+      R[0][0] = cp[0]; R[1][1] = cp[1]; R[2][2] = cp[2];
+      dR[0][0][0] = 1.0; dR[1][1][1] = 1.0; dR[2][2][2] = 1.0;
+    } else if (cs == "hexagonal") {
+      need(2);
+      const double a = cp[0], c = cp[1];
+      R[0][0] = a; R[1][0] = -0.5 * a; R[1][1] = a * std::sqrt(0.75); R[2][2] = c;
+      dR[0][0][0] = 1.0; dR[1][0][0] = -0.5; dR[1][1][0] = std::sqrt(0.75); dR[2][2][1] = 1.0;
+    } else {
+      throw std::runtime_error("crystal system not supported in 3D: " + cs);
+    }
+  } else if (dim == 2) {
+    if (cs == "square") {
+      need(1);
+      R[0][0] = cp[0]; R[1][1] = cp[0];
+      dR[0][0][0] = 1.0; dR[1][1][0] = 1.0;
+    } else if (cs == "rectangular") {
+      need(2);
+      R[0][0] = cp[0]; R[1][1] = cp[1];
+      dR[0][0][0] = 1.0; dR[1][1][1] = 1.0;
+    } else if (cs == "hexagonal") {
+      need(1);
+      const double a = cp[0];
+      R[0][0] = a; R[1][0] = -0.5 * a; R[1][1] = a * std::sqrt(0.75);
+      dR[0][0][0] = 1.0; dR[1][0][0] = -0.5; dR[1][1][0] = std::sqrt(0.75);
+    } else {
+      throw std::runtime_error("crystal system not supported in 2D: " + cs);
+    }
+  } else if (dim == 1) {
+    if (cs != "lamellar") throw std::runtime_error("Only 1D system is \"lamellar\"");
+    R[0][0] = cp[0];
+    dR[0][0][0] = 1.0;
+  } else {
+    throw std::runtime_error("Invalid dimension");
+  }
+  const double twopi = 4.0 * std::acos(0.0);
+  double Rl[3][3] = {{0}}, Ri[3][3] = {{0}};
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) Rl[i][j] = R[i][j];
+  inverse3(Rl, Ri, dim);
+  std::memset(G, 0, sizeof(double) * 9);
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) G[i][j] = twopi * Ri[j][i];
+  double dG[3][3][6] = {{{0}}};
+  for (int k = 0; k < ncp; ++k)
+    for (int i = 0; i < dim; ++i)
+      for (int j = 0; j < dim; ++j) {
+        for (int l = 0; l < dim; ++l)
+          for (int m = 0; m < dim; ++m) dG[i][j][k] = dG[i][j][k] - G[i][l] * dR[m][l][k] * G[m][j];
+        dG[i][j][k] = dG[i][j][k] / twopi;
+      }
+  for (int k = 0; k < ncp; ++k)
+    for (int i = 0; i < dim; ++i)
+      for (int j = 0; j < dim; ++j)
+        for (int l = 0; l < dim; ++l)
+          dGG[i][j][k] = dGG[i][j][k] + G[i][l] * dG[j][l][k] + G[j][l] * dG[i][l][k];
+}
+
+// ------------------------------------------------------- Anderson mixing
+// State of iterate_AM (src/iterate_mod.fp.f90:1042-1490) for one system.
+struct AM {
+  int nstar = 0, nmon = 0, ncp = 0, nhist = 0, domain = 0, itr = 0;
+  int n1 = 0;                                  // N_star - 1 coefficients per monomer
+  std::vector<double> dev, om_hist, dev_cp, cp_hist;   // [h][a][k], [h][c]
+  double* d(int h, int a) { return &dev[((size_t)h * nmon + a) * n1]; }
+  double* o(int h, int a) { return &om_hist[((size_t)h * nmon + a) * n1]; }
+};
+
+// LU with partial pivoting, then solve U x = ... for the n right-hand side
+// Vm; returns Cn = Umn^{-1} Vm (dgetrf/dgetri + matmul at :1384-1387).
+std::vector<double> solve(std::vector<double> A, std::vector<double> b, int n) {
+  std::vector<int> piv(n);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[r * n + c]) > std::fabs(A[p * n + c])) p = r;
+    if (A[p * n + c] == 0.0) throw std::runtime_error("Full Umn LU factorization failed.");
+    if (p != c) {
+      for (int j = 0; j < n; ++j) std::swap(A[p * n + j], A[c * n + j]);
+      std::swap(b[p], b[c]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r * n + c] / A[c * n + c];
+      for (int j = c; j < n; ++j) A[r * n + j] -= f * A[c * n + j];
+      b[r] -= f * b[c];
+    }
+  }
+  std::vector<double> x(n);
+  for (int r = n - 1; r >= 0; --r) {
+    double s = b[r];
+    for (int j = r + 1; j < n; ++j) s -= A[r * n + j] * x[j];
+    x[r] = s / A[r * n + r];
+  }
+  return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pscf_am_last_error(void) { return g_am_err.c_str(); }
+
+int pscf_unit_cell(int dim, const char* crystal_system, int ncp, const double* cell_param,
+                   double* R_basis, double* G_basis, double* dGG_basis) {
+  return guard([&] {
+    double R[3][3], G[3][3], dGG[3][3][6];
+    unit_cell(dim, crystal_system, ncp, cell_param, R, G, dGG);
+    // Fortran layouts: R(dim,dim), G(dim,dim), dGG(dim,dim,6)
+    for (int i = 0; i < dim; ++i)
+      for (int j = 0; j < dim; ++j) {
+        if (R_basis) R_basis[i + dim * j] = R[i][j];
+        if (G_basis) G_basis[i + dim * j] = G[i][j];
+        for (int k = 0; k < 6; ++k)
+          if (dGG_basis) dGG_basis[i + dim * (j + dim * k)] = dGG[i][j][k];
+      }
+  });
+}
+
+int pscf_make_dgsq(int dim, int n_star, const int* wave_of_star, const double* dGG, double* dGsq) {
+  // src/basis_mod.f90:1993-2014; dGG(dim,dim), wave_of_star(dim,N_star)
+  return guard([&] {
+    for (int s = 0; s < n_star; ++s) {
+      double acc = 0.0;
+      for (int j = 0; j < dim; ++j)
+        for (int k = 0; k < dim; ++k)
+          acc = acc + (double)wave_of_star[j + dim * s] * (double)wave_of_star[k + dim * s] *
+                          dGG[j + dim * k];
+      dGsq[s] = acc;
+    }
+  });
+}
+
+int pscf_am_create(int n_star, int n_monomer, int n_cell_param, int n_hist, int domain,
+                   int64_t* handle) {
+  return guard([&] {
+    if (n_monomer != 2 && n_monomer != 3)
+      throw std::runtime_error("Anderson Mixing is only implemented for N_monomer = 2 or 3");
+    auto* a = new AM;
+    a->nstar = n_star;
+    a->nmon = n_monomer;
+    a->ncp = n_cell_param;
+    a->nhist = n_hist;
+    a->domain = domain;
+    a->n1 = n_star - 1;
+    const size_t H = (size_t)n_hist + 1;
+    a->dev.assign(H * n_monomer * a->n1, 0.0);
+    a->om_hist.assign(H * n_monomer * a->n1, 0.0);
+    a->dev_cp.assign(H * n_cell_param, 0.0);
+    a->cp_hist.assign(H * n_cell_param, 0.0);
+    *handle = reinterpret_cast<int64_t>(a);
+  });
+}
+
+int pscf_am_destroy(int64_t handle) {
+  return guard([&] { delete reinterpret_cast<AM*>(handle); });
+}
+
+// iterate_AM's history start (:1135-1141): omega already has its uniform
+// component set (set_omega_uniform); records om_hist(1), cp_hist(1).
+int pscf_am_start(int64_t handle, const double* omega, const double* cell_param) {
+  return guard([&] {
+    AM& A = *reinterpret_cast<AM*>(handle);
+    A.itr = 0;
+    std::fill(A.dev.begin(), A.dev.end(), 0.0);
+    std::fill(A.om_hist.begin(), A.om_hist.end(), 0.0);
+    std::fill(A.dev_cp.begin(), A.dev_cp.end(), 0.0);
+    std::fill(A.cp_hist.begin(), A.cp_hist.end(), 0.0);
+    for (int a = 0; a < A.nmon; ++a)
+      for (int k = 0; k < A.n1; ++k) A.o(0, a)[k] = omega[a + (size_t)A.nmon * (k + 1)];
+    if (A.domain)
+      for (int c = 0; c < A.ncp; ++c) A.cp_hist[c] = cell_param[c];
+  });
+}
+
+// One pass of iterate_AM's loop after density (and scf_stress when domain):
+// residuals (:1171-1219), convergence test (:1241-1250) and, when not
+// converged, the mixing update of omega(:,2:) and cell_param (:1252-1446).
+// status: 0 updated, 1 converged, 2 itr > max_itr, 3 diverged.
+int pscf_am_iterate(int64_t handle, double* omega, const double* rho, const double* chi,
+                    const double* stress, double* cell_param, double error_max, int max_itr,
+                    double* error_out, int* status) {
+  return guard([&] {
+    AM& A = *reinterpret_cast<AM*>(handle);
+    const int nm = A.nmon, n1 = A.n1, ncp = A.ncp;
+    A.itr += 1;
+    const int itr = A.itr;
+    int itr_prev;
+    double lam;
+    if (itr < A.nhist + 1) {
+      itr_prev = itr - 1;
+      // 1-0.9**itr is evaluated in default (single) precision in the reference
+      float p = 1.0f;
+      for (int i = 0; i < itr; ++i) p = p * 0.9f;
+      lam = (double)(1.0f - p);
+    } else {
+      lam = 1.0;
+      itr_prev = A.nhist;
+    }
+    const int cur = itr_prev;        // 0-based index of dev(itr_prev+1)
+    auto om = [&](int a, int k) -> double& { return omega[a + (size_t)nm * (k + 1)]; };
+    auto rh = [&](int a, int k) { return rho[a + (size_t)nm * (k + 1)]; };
+    auto ch = [&](int a, int b) { return chi[a + (size_t)nm * b]; };
+    for (int a = 0; a < nm; ++a) {
+      double* dv = A.d(cur, a);
+      for (int k = 0; k < n1; ++k) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int b = 0; b < nm; ++b) s1 += ch(b, a) * rh(b, k);
+        for (int b = 0; b < nm; ++b) s2 += om(b, k);
+        if (nm == 2) {
+          dv[k] = s1 + s2 / nm - om(a, k);
+        } else {
+          dv[k] = s1 + (s2 + ch(0, 1) * rh(2, k) + ch(1, 2) * rh(0, k) + ch(0, 2) * rh(1, k)) / nm -
+                  om(a, k);
+        }
+      }
+    }
+    if (A.domain)
+      for (int c = 0; c < ncp; ++c) A.dev_cp[(size_t)cur * ncp + c] = -stress[c];
+    double e1 = 0.0, e2 = 0.0;
+    for (int a = 0; a < nm; ++a)
+      for (int k = 0; k < n1; ++k) e1 = std::max(e1, std::fabs(A.d(cur, a)[k]));
+    double error = e1;
+    if (A.domain) {
+      for (int c = 0; c < ncp; ++c) e2 = std::max(e2, std::fabs(A.dev_cp[(size_t)cur * ncp + c]));
+      error = std::max(e1, e2 * 1.0);
+    }
+    *error_out = error;
+    if (error < error_max) { *status = 1; return; }
+    if (itr > max_itr) { *status = 2; return; }
+    if (error > 100.0 * 100.0) { *status = 3; return; }
+
+    std::vector<double> WW(n1), DD(n1);
+    if (itr == 1) {
+      for (int a = 0; a < nm; ++a)
+        for (int k = 0; k < n1; ++k) om(a, k) = om(a, k) + lam * A.d(0, a)[k];
+      if (A.domain)
+        for (int c = 0; c < ncp; ++c) cell_param[c] = cell_param[c] + lam * A.dev_cp[c];
+    } else {
+      const int np = itr_prev;
+      std::vector<double> U((size_t)np * np, 0.0), V(np, 0.0);
+      for (int m = 1; m <= np; ++m) {
+        for (int n = m; n <= np; ++n) {
+          double u = 0.0;
+          for (int a = 0; a < nm; ++a) {
+            const double *dc = A.d(cur, a), *dm = A.d(cur - m, a), *dn = A.d(cur - n, a);
+            double s = 0.0;
+            for (int k = 0; k < n1; ++k) s += (dc[k] - dm[k]) * (dc[k] - dn[k]);
+            u += s;
+          }
+          if (A.domain) {
+            double s = 0.0;
+            for (int c = 0; c < ncp; ++c)
+              s += (A.dev_cp[(size_t)cur * ncp + c] - A.dev_cp[(size_t)(cur - m) * ncp + c]) *
+                   (A.dev_cp[(size_t)cur * ncp + c] - A.dev_cp[(size_t)(cur - n) * ncp + c]);
+            u += s;
+          }
+          U[(size_t)(m - 1) * np + (n - 1)] = u;
+          U[(size_t)(n - 1) * np + (m - 1)] = u;
+        }
+        double v = 0.0;
+        for (int a = 0; a < nm; ++a) {
+          const double *dc = A.d(cur, a), *dm = A.d(cur - m, a);
+          double s = 0.0;
+          for (int k = 0; k < n1; ++k) s += (dc[k] - dm[k]) * dc[k];
+          v += s;
+        }
+        if (A.domain) {
+          double s = 0.0;
+          for (int c = 0; c < ncp; ++c)
+            s += (A.dev_cp[(size_t)cur * ncp + c] - A.dev_cp[(size_t)(cur - m) * ncp + c]) *
+                 A.dev_cp[(size_t)cur * ncp + c];
+          v += s;
+        }
+        V[m - 1] = v;
+      }
+      std::vector<double> Cn = np == 1 ? std::vector<double>{V[0] / U[0]} : solve(U, V, np);
+      for (int a = 0; a < nm; ++a) {
+        for (int k = 0; k < n1; ++k) {
+          WW[k] = om(a, k);
+          DD[k] = A.d(cur, a)[k];
+        }
+        for (int n = 1; n <= np; ++n) {
+          const double *oh = A.o(cur - n, a), *oc = A.o(cur, a);
+          const double *dn = A.d(cur - n, a), *dc = A.d(cur, a);
+          for (int k = 0; k < n1; ++k) {
+            WW[k] = WW[k] + Cn[n - 1] * (oh[k] - oc[k]);
+            DD[k] = DD[k] + Cn[n - 1] * (dn[k] - dc[k]);
+          }
+        }
+        for (int k = 0; k < n1; ++k) om(a, k) = WW[k] + lam * DD[k];
+      }
+      if (A.domain) {
+        for (int c = 0; c < ncp; ++c) {
+          double w = cell_param[c], dd = A.dev_cp[(size_t)cur * ncp + c];
+          for (int n = 1; n <= np; ++n) {
+            w = w + Cn[n - 1] * (A.cp_hist[(size_t)(cur - n) * ncp + c] - A.cp_hist[(size_t)cur * ncp + c]);
+            dd = dd + Cn[n - 1] * (A.dev_cp[(size_t)(cur - n) * ncp + c] - A.dev_cp[(size_t)cur * ncp + c]);
+          }
+          cell_param[c] = w + lam * dd;
+        }
+      }
+    }
+    // history (:1429-1446)
+    if (itr < A.nhist + 1) {
+      for (int a = 0; a < nm; ++a)
+        for (int k = 0; k < n1; ++k) A.o(itr, a)[k] = om(a, k);
+      if (A.domain)
+        for (int c = 0; c < ncp; ++c) A.cp_hist[(size_t)itr * ncp + c] = cell_param[c];
+    } else {
+      const size_t blk = (size_t)nm * n1;
+      std::memmove(A.dev.data(), A.dev.data() + blk, sizeof(double) * blk * A.nhist);
+      std::memmove(A.om_hist.data(), A.om_hist.data() + blk, sizeof(double) * blk * A.nhist);
+      for (int a = 0; a < nm; ++a)
+        for (int k = 0; k < n1; ++k) A.o(A.nhist, a)[k] = om(a, k);
+      if (A.domain) {
+        std::memmove(A.dev_cp.data(), A.dev_cp.data() + ncp, sizeof(double) * ncp * A.nhist);
+        std::memmove(A.cp_hist.data(), A.cp_hist.data() + ncp, sizeof(double) * ncp * A.nhist);
+        for (int c = 0; c < ncp; ++c) A.cp_hist[(size_t)A.nhist * ncp + c] = cell_param[c];
+      }
+    }
+    *status = 0;
+  });
+}
+
+}  // extern "C"

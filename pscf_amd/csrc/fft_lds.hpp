@@ -1,0 +1,265 @@
+// LDS-resident FP64 FFT building blocks for gfx950.
+//
+// Lines of length N (N = 2^a 3^b 5^c) are transformed in place in LDS by a
+// mixed-radix Stockham schedule (radix 8, then 4, 2, 3, 5).  Each pass loads
+// every butterfly's inputs into registers, synchronises the workgroup, then
+// writes the outputs to their Stockham positions, so a pass needs no second
+// buffer.  Sign convention: SIGN = -1 is the forward transform
+// X[k] = sum_n x[n] e^{-2 pi i k n / N}, SIGN = +1 the inverse; both are
+// unnormalised, matching the FFTW calls of the reference (src/fft3_mod.f90:85-122).
+//
+// Twiddles come from a per-length table tw[m] = exp(-2 pi i m / N), m < N,
+// built on the host; SIGN = +1 uses its conjugate.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pscf {
+
+// ---------------------------------------------------------------- complex ops
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by s*i, s = +-1
+template <int S> __device__ __forceinline__ double2 cmul_si(double2 a) {
+  return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// ------------------------------------------------------ compile-time factoring
+constexpr int fft_nfac(int n) {
+  int c = 0;
+  while (n > 1 && n % 8 == 0) { n /= 8; ++c; }
+  while (n % 4 == 0) { n /= 4; ++c; }
+  while (n % 2 == 0) { n /= 2; ++c; }
+  while (n % 3 == 0) { n /= 3; ++c; }
+  while (n % 5 == 0) { n /= 5; ++c; }
+  return n == 1 ? c : -1;
+}
+constexpr int fft_fac(int n, int p) {
+  int c = 0;
+  while (n > 1 && n % 8 == 0) { if (c == p) return 8; n /= 8; ++c; }
+  while (n % 4 == 0) { if (c == p) return 4; n /= 4; ++c; }
+  while (n % 2 == 0) { if (c == p) return 2; n /= 2; ++c; }
+  while (n % 3 == 0) { if (c == p) return 3; n /= 3; ++c; }
+  while (n % 5 == 0) { if (c == p) return 5; n /= 5; ++c; }
+  return 0;
+}
+
+// ------------------------------------------------------------ radix kernels
+template <int R, int S> struct Dft;
+
+template <int S> struct Dft<2, S> {
+  static __device__ __forceinline__ void run(double2* v) {
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int S> struct Dft<4, S> {
+  static __device__ __forceinline__ void run(double2* v) {
+    double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    double2 t2 = cadd(v[1], v[3]), t3 = cmul_si<S>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <int S> struct Dft<8, S> {
+  static __device__ __forceinline__ void run(double2* v) {
+    constexpr double h = 0.70710678118654752440;  // 1/sqrt(2)
+    double2 e[4] = {v[0], v[2], v[4], v[6]};
+    double2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, S>::run(e);
+    Dft<4, S>::run(o);
+    // o[q] *= w8^q, w8 = (1 + S i)/sqrt(2)
+    double2 o1 = make_double2(h * (o[1].x - S * o[1].y), h * (o[1].y + S * o[1].x));
+    double2 o2 = cmul_si<S>(o[2]);
+    double2 o3 = make_double2(h * (-o[3].x - S * o[3].y), h * (-o[3].y + S * o[3].x));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+template <int S> struct Dft<3, S> {
+  static __device__ __forceinline__ void run(double2* v) {
+    constexpr double r3 = 0.86602540378443864676;  // sqrt(3)/2
+    double2 t = cadd(v[1], v[2]);
+    double2 d = csub(v[1], v[2]);
+    double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+    double2 sd = cmul_si<S>(cscale(d, r3));
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, sd);
+    v[2] = csub(m, sd);
+  }
+};
+
+template <int S> struct Dft<5, S> {
+  static __device__ __forceinline__ void run(double2* v) {
+    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    constexpr double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    double2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    double2 b1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+    double2 b2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+    double2 ib1 = cmul_si<S>(b1), ib2 = cmul_si<S>(b2);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, ib1);
+    v[4] = csub(a1, ib1);
+    v[2] = cadd(a2, ib2);
+    v[3] = csub(a2, ib2);
+  }
+};
+
+// ----------------------------------------------------------- Stockham pass
+// LINES lines of length N in `buf`; element n of line l sits at
+// buf[l*LS + n*ES].  JFAST selects the thread mapping: consecutive threads
+// take consecutive butterflies of one line (rows, ES == 1) or the same
+// butterfly of consecutive lines (columns, LS == 1), so LDS addresses of a
+// wavefront are consecutive either way.
+template <int N, int R, int NS, int SIGN, int LINES, int LS, int ES, bool JFAST, int NT>
+__device__ __forceinline__ void stockham_pass(double2* buf, const double2* __restrict__ tw, int tid) {
+  constexpr int NB = N / R;
+  constexpr int TOT = LINES * NB;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  constexpr int TSTEP = N / (NS * R);
+  double2 v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int bf = tid + p * NT;
+    if (bf < TOT) {
+      const int j = JFAST ? bf % NB : bf / LINES;
+      const int line = JFAST ? bf / NB : bf % LINES;
+      const double2* base = buf + line * LS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[p][r] = base[(j + r * NB) * ES];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int bf = tid + p * NT;
+    if (bf < TOT) {
+      const int j = JFAST ? bf % NB : bf / LINES;
+      const int line = JFAST ? bf / NB : bf % LINES;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          double2 w = tw[k * r * TSTEP];
+          if (SIGN > 0) w.y = -w.y;
+          v[p][r] = cmul(v[p][r], w);
+        }
+      }
+      Dft<R, SIGN>::run(v[p]);
+      double2* base = buf + line * LS;
+      const int idx = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) base[(idx + q * NS) * ES] = v[p][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int P, int NS, int SIGN, int LINES, int LS, int ES, bool JFAST, int NT>
+__device__ __forceinline__ void fft_passes(double2* buf, const double2* __restrict__ tw, int tid) {
+  if constexpr (P < fft_nfac(N)) {
+    constexpr int R = fft_fac(N, P);
+    stockham_pass<N, R, NS, SIGN, LINES, LS, ES, JFAST, NT>(buf, tw, tid);
+    fft_passes<N, P + 1, NS * R, SIGN, LINES, LS, ES, JFAST, NT>(buf, tw, tid);
+  }
+}
+
+// In-place FFT of LINES lines of length N in LDS (a no-op for N == 1).
+template <int N, int SIGN, int LINES, int LS, int ES, bool JFAST, int NT>
+__device__ __forceinline__ void fft_lines(double2* buf, const double2* __restrict__ tw, int tid) {
+  static_assert(fft_nfac(N) >= 0, "FFT length must be 2^a 3^b 5^c");
+  fft_passes<N, 0, 1, SIGN, LINES, LS, ES, JFAST, NT>(buf, tw, tid);
+}
+
+// ----------------------------------------------------------- plane helpers
+// A real plane of N1 x N2 points (x fastest) is packed two rows per complex
+// line: P[pl*N1 + x] = row(2 pl)[x] + i row(2 pl + 1)[x] (the odd partner of
+// an odd last row is 0).  After the forward x-transform each pair is split
+// into the two half spectra (k = 0..N1/2) of a half plane
+// H[y*(N1/2+1) + k]; the inverse rebuilds the pairs from the Hermitian
+// extension, using only the real parts at k = 0 and k = N1/2 as a c2r
+// transform does.
+template <int N1, int N2> struct Plane {
+  static constexpr int N1H = N1 / 2 + 1;
+  static constexpr int NP = (N2 + 1) / 2;        // packed row pairs
+  static constexpr int PSIZE = N1 * NP;          // complex words in P
+  static constexpr int HSIZE = N1H * N2;         // complex words in H
+  static_assert(N1 % 2 == 0, "first grid dimension must be even");
+};
+
+// P (x-transformed pairs) -> H (half rows)
+template <int N1, int N2, int NT>
+__device__ __forceinline__ void split_pairs(const double2* P, double2* H, int tid) {
+  using PL = Plane<N1, N2>;
+  constexpr int TOT = PL::NP * PL::N1H;
+  for (int i = tid; i < TOT; i += NT) {
+    const int pl = i / PL::N1H, k = i % PL::N1H;
+    const double2 z = P[pl * N1 + k];
+    const double2 zc = cconj(P[pl * N1 + (N1 - k) % N1]);
+    H[(2 * pl) * PL::N1H + k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
+    if (2 * pl + 1 < N2) {
+      const double2 d = csub(z, zc);                       // (z - zc) / (2i)
+      H[(2 * pl + 1) * PL::N1H + k] = make_double2(0.5 * d.y, -0.5 * d.x);
+    }
+  }
+  __syncthreads();
+}
+
+// H (half rows, after the inverse y-transform) -> P (pairs ready for the inverse x-transform)
+template <int N1, int N2, int NT>
+__device__ __forceinline__ void join_pairs(const double2* H, double2* P, int tid) {
+  using PL = Plane<N1, N2>;
+  constexpr int TOT = PL::NP * N1;
+  for (int i = tid; i < TOT; i += NT) {
+    const int pl = i / N1, x = i % N1;
+    const bool hi = x > N1 / 2;
+    const int k = hi ? N1 - x : x;
+    double2 a = H[(2 * pl) * PL::N1H + k];
+    double2 b = (2 * pl + 1 < N2) ? H[(2 * pl + 1) * PL::N1H + k] : make_double2(0.0, 0.0);
+    if (k == 0 || k == N1 / 2) { a.y = 0.0; b.y = 0.0; }
+    if (hi) { a.y = -a.y; b.y = -b.y; }
+    P[pl * N1 + x] = make_double2(a.x - b.y, a.y + b.x);     // a + i b
+  }
+  __syncthreads();
+}
+
+// Forward r2c of the plane held in P (pairs, already filled): x then y.
+template <int N1, int N2, int NT>
+__device__ __forceinline__ void plane_forward(double2* P, double2* H, const double2* tw1,
+                                              const double2* tw2, int tid) {
+  using PL = Plane<N1, N2>;
+  fft_lines<N1, -1, PL::NP, N1, 1, true, NT>(P, tw1, tid);
+  split_pairs<N1, N2, NT>(P, H, tid);
+  fft_lines<N2, -1, PL::N1H, 1, PL::N1H, false, NT>(H, tw2, tid);
+}
+
+// Inverse c2r of the half plane held in H: y then x; result pairs in P.
+template <int N1, int N2, int NT>
+__device__ __forceinline__ void plane_inverse(double2* H, double2* P, const double2* tw1,
+                                              const double2* tw2, int tid) {
+  using PL = Plane<N1, N2>;
+  fft_lines<N2, +1, PL::N1H, 1, PL::N1H, false, NT>(H, tw2, tid);
+  join_pairs<N1, N2, NT>(H, P, tid);
+  fft_lines<N1, +1, PL::NP, N1, 1, true, NT>(P, tw1, tid);
+}
+
+}  // namespace pscf

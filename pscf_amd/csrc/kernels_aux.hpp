@@ -1,0 +1,302 @@
+// Pointwise, gather/scatter and reduction kernels of the SCFT inner loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+
+namespace pscf {
+
+constexpr int NTA = 256;
+
+// ----------------------------------------------------------------------------
+// Minimum-image wavevectors and |G|^2 on the half grid
+// (src/grid_mod.f90:141-160, 225-283, 402-413).  Trial shifts +N, 0, -N per
+// dimension in that order; a trial replaces the best only if its |G|^2 is
+// smaller by more than 1e-8.  Arithmetic is kept unfused so |G|^2 rounds as
+// the reference's sums do.  Outputs: ksq(kx,ky,kz) and, through the dense
+// wave table wtab (index (G - gmin) over a box), the 1-based wave number
+// (0 if the wave is not in the basis), as basis_to_kgrid looks it up.
+// ----------------------------------------------------------------------------
+struct GeomArgs {
+  int n1, n2, n3, dim;
+  double gb[9];          // G_basis(i, j) at gb[i*3 + j] (row i = i-th reciprocal vector)
+  int gmin[3], gext[3];  // dense wave table box: G in [gmin, gmin + gext)
+};
+
+__global__ void __launch_bounds__(NTA) k_g_to_bz(GeomArgs g, const int* __restrict__ wtab,
+                                                 double* __restrict__ ksq, int* __restrict__ kwave) {
+#pragma clang fp contract(off)
+  const int n1h = g.n1 / 2 + 1;
+  const long mk = (long)n1h * g.n2 * g.n3;
+  for (long idx = blockIdx.x * (long)NTA + threadIdx.x; idx < mk; idx += (long)gridDim.x * NTA) {
+    const int G[3] = {(int)(idx % n1h), (int)((idx / n1h) % g.n2), (int)(idx / ((long)n1h * g.n2))};
+    const int ng[3] = {g.n1, g.n2, g.n3};
+    int best[3] = {0, 0, 0};
+    double bsq = 1.0e10;
+    const int nt = g.dim == 1 ? 3 : (g.dim == 2 ? 9 : 27);
+    for (int t = 0; t < nt; ++t) {
+      int sh[3] = {0, 0, 0};
+      int r = t;
+      for (int d = g.dim - 1; d >= 0; --d) { sh[d] = 1 - r % 3; r /= 3; }
+      int Gt[3] = {0, 0, 0};
+      for (int d = 0; d < g.dim; ++d) Gt[d] = G[d] + sh[d] * ng[d];
+      double gsq = 0.0;
+      double gv[3];
+      for (int jj = 0; jj < g.dim; ++jj) {
+        double s = 0.0;
+        for (int ii = 0; ii < g.dim; ++ii) s = s + (double)Gt[ii] * g.gb[ii * 3 + jj];
+        gv[jj] = s;
+      }
+      for (int jj = 0; jj < g.dim; ++jj) gsq = gsq + gv[jj] * gv[jj];
+      if (gsq < bsq - 1.0e-8) {
+        bsq = gsq;
+        for (int d = 0; d < g.dim; ++d) best[d] = Gt[d];
+      }
+    }
+    ksq[idx] = bsq;
+    int w = 0;
+    bool inside = true;
+    long off = 0, mul = 1;
+    for (int d = 0; d < g.dim; ++d) {
+      const int u = best[d] - g.gmin[d];
+      if (u < 0 || u >= g.gext[d]) inside = false;
+      off += (long)u * mul;
+      mul *= g.gext[d];
+    }
+    if (inside) w = wtab[off];
+    kwave[idx] = w;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Star basis -> half k grid (src/grid_basis_mod.f90:47-129), nf fields.
+// kwave: 1-based wave per half-grid point (0 = none); coeff/star_of_wave/
+// star_invert: basis_mod tables (src/basis_mod.f90:60-84).
+// basis(nf, N_star) is the reference's omega(N_monomer, N_star) layout.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_basis_to_kgrid(const double* __restrict__ basis, int nf,
+                                                        int nstar, const int* __restrict__ kwave,
+                                                        const double2* __restrict__ coeff,
+                                                        const int* __restrict__ star_of_wave,
+                                                        const int* __restrict__ star_invert, long mk,
+                                                        double2* __restrict__ out, int no_parity) {
+  const double sq2 = 1.41421356237309514547;  // dsqrt(2.0_long)
+  const long n = mk * nf;
+  for (long gi = blockIdx.x * (long)NTA + threadIdx.x; gi < n; gi += (long)gridDim.x * NTA) {
+    const int f = (int)(gi / mk);
+    const long idx = gi % mk;
+    const int w = kwave[idx];
+    double2 z = make_double2(0.0, 0.0);
+    if (w >= 1) {
+      const double2 c = coeff[w - 1];
+      const int s = star_of_wave[w - 1] - 1;
+      const int inv = no_parity ? 0 : star_invert[s];
+      if (inv == 0) {
+        const double a = basis[(long)s * nf + f];
+        z = make_double2(a * c.x, a * c.y);
+      } else {
+        double a, bb;
+        if (inv == 1) { a = basis[(long)s * nf + f]; bb = -basis[(long)(s + 1) * nf + f]; }
+        else          { a = basis[(long)(s - 1) * nf + f]; bb = basis[(long)s * nf + f]; }
+        z = cmul(make_double2(a, bb), c);
+        z = make_double2(z.x / sq2, z.y / sq2);
+      }
+    }
+    out[gi] = z;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Half k grid -> star basis (src/grid_basis_mod.f90:146-200) for nf fields,
+// then times `scale` (density() divides by M: src/scf_mod.fp.f90:224).
+// sidx: half-grid index of the star's representative wave, sconj: 1 if the
+// representative lies in kx > N1/2 and is read as conj of -k.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_kgrid_to_basis(const double2* __restrict__ kg, long mk,
+                                                        int nf, int nstar,
+                                                        const long* __restrict__ sidx,
+                                                        const int* __restrict__ sconj,
+                                                        const double2* __restrict__ scoef,
+                                                        const int* __restrict__ star_invert,
+                                                        double scale, double* __restrict__ basis,
+                                                        int no_parity) {
+  const double sq2 = 1.41421356237309514547;
+  const long n = (long)nstar * nf;
+  for (long gi = blockIdx.x * (long)NTA + threadIdx.x; gi < n; gi += (long)gridDim.x * NTA) {
+    const int f = (int)(gi % nf);
+    const int s = (int)(gi / nf);
+    double2 z = kg[f * mk + sidx[s]];
+    if (sconj[s] && !no_parity) z.y = -z.y;
+    const double2 c = scoef[s];
+    // z / c
+    const double den = c.x * c.x + c.y * c.y;
+    const double2 q = make_double2((z.x * c.x + z.y * c.y) / den, (z.y * c.x - z.x * c.y) / den);
+    const int inv = no_parity ? 0 : star_invert[s];
+    double v = inv == 0 ? q.x : (inv == 1 ? q.x * sq2 : q.y * sq2);
+    basis[(long)s * nf + f] = v * scale;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// make_propg (src/step_mod.f90:109-129) for one block
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_make_propg(double ds, double b, const double* __restrict__ ksq,
+                                                    long mk, const double* __restrict__ omega, long m,
+                                                    double* __restrict__ e1, double* __restrict__ e2,
+                                                    double* __restrict__ k1, double* __restrict__ k2) {
+  const double lap = ds * (b * b) / 6.0;
+  const double pot = ds / 2.0;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < m; i += (long)gridDim.x * NTA) {
+    if (i < mk) {
+      k1[i] = exp(-(lap * ksq[i]));
+      k2[i] = exp(-(lap / 2.0 * ksq[i]));
+    }
+    e1[i] = exp(-(pot * omega[i]));
+    e2[i] = exp(-(pot / 2.0 * omega[i]));
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Block partial sums of x (n values); out[blockIdx.x]
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = NTA / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(NTA) k_partial_sum(const double* __restrict__ x, long n,
+                                                     double* __restrict__ part) {
+  __shared__ double red[NTA];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA) acc += x[i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[slot] = sum(part[0..np)) * scale
+__global__ void __launch_bounds__(NTA) k_final_sum(const double* __restrict__ part, int np,
+                                                   double scale, double* __restrict__ out) {
+  __shared__ double red[NTA];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += NTA) acc += part[i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[0] = s * scale;
+}
+
+// solvent Boltzmann factor partial sums: sum exp(-size*w)
+__global__ void __launch_bounds__(NTA) k_partial_sum_exp(const double* __restrict__ w, double size,
+                                                         long n, double* __restrict__ part) {
+  __shared__ double red[NTA];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA)
+    acc += exp((-size) * w[i]);
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ----------------------------------------------------------------------------
+// Solvent density (src/scf_mod.fp.f90:251-295):
+//   rho += phi_s exp(-size w)/Q_s, with phi_s = Q_s e^{mu_s} in the grand ensemble.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_solvent_rho(const double* __restrict__ w, double size,
+                                                     const double* __restrict__ qs, int ensemble,
+                                                     double phi, double mu, long n,
+                                                     double* __restrict__ rho) {
+  const double Q = qs[0];
+  const double ph = ensemble == 1 ? Q * exp(mu) : phi;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA)
+    rho[i] = rho[i] + ph * exp((-size) * w[i]) / Q;
+}
+
+// ----------------------------------------------------------------------------
+// Simpson contour integration of one chain (src/scf_mod.fp.f90:362-424, 212-218):
+//   rho_b = ds_b/3 [qf qr(ibgn) + qf qr(iend) + 4 sum_odd + 2 sum_even]
+//   rho_grid(monomer_b) += phi * rho_b / N / Q
+// in the reference's summation order per point.
+// ----------------------------------------------------------------------------
+struct SimpsonArgs {
+  int nblk;
+  int bnd[17];           // 0-based contour index of block starts, bnd[nblk] = ns
+  double ds[16];
+  int monomer[16];       // 0-based
+  double chain_length;
+  double phi, mu;
+  int ensemble;
+};
+
+__global__ void __launch_bounds__(NTA) k_simpson(const double* __restrict__ qf,
+                                                 const double* __restrict__ qr, long m,
+                                                 SimpsonArgs a, const double* __restrict__ Qp,
+                                                 double* __restrict__ rho_grid) {
+  const double Q = Qp[0];
+  const double phi = a.ensemble == 1 ? Q * exp(a.mu) : a.phi;
+  for (long r = blockIdx.x * (long)NTA + threadIdx.x; r < m; r += (long)gridDim.x * NTA) {
+    for (int b = 0; b < a.nblk; ++b) {
+      const int ibgn = a.bnd[b], iend = a.bnd[b + 1];
+      double acc = qf[(long)ibgn * m + r] * qr[(long)ibgn * m + r];
+      acc = acc + qf[(long)iend * m + r] * qr[(long)iend * m + r];
+      for (int jj = ibgn + 1; jj <= iend - 1; jj += 2)
+        acc = acc + qf[(long)jj * m + r] * qr[(long)jj * m + r] * 4.0;
+      for (int jj = ibgn + 2; jj <= iend - 2; jj += 2)
+        acc = acc + qf[(long)jj * m + r] * qr[(long)jj * m + r] * 2.0;
+      acc = acc * a.ds[b] / 3.0;
+      acc = acc / a.chain_length / Q;
+      double* rg = rho_grid + (long)a.monomer[b] * m;
+      rg[r] = rg[r] + phi * acc;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Stress (src/scf_mod.fp.f90:505-530): for each contour point p of a chunk,
+// dot(p, beta) = sum_star qr_basis(star) * dGsq(star, beta) * qf_basis(star).
+// qfb/qrb hold star coefficients as (np, N_star) Fortran order (p fastest),
+// the layout k_kgrid_to_basis writes; dgsq is (N_star, ncp).  One workgroup
+// per point; out(p, beta) at out[p*ncp + beta].
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_stress_dots(const double* __restrict__ qfb,
+                                                     const double* __restrict__ qrb, int np,
+                                                     int nstar, const double* __restrict__ dgsq,
+                                                     int ncp, double* __restrict__ out) {
+  __shared__ double red[NTA];
+  const int p = blockIdx.x;
+  for (int beta = 0; beta < ncp; ++beta) {
+    double acc = 0.0;
+    for (int s = threadIdx.x; s < nstar; s += NTA)
+      acc += qrb[(long)s * np + p] * dgsq[(long)beta * nstar + s] * qfb[(long)s * np + p];
+    const double v = block_sum(acc, red);
+    if (threadIdx.x == 0) out[(long)p * ncp + beta] = v;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// divide_energy head/tail terms (src/scf_mod.fp.f90:631-662): partial sums of
+// sum_r [qf>0 & qr>0] qf qr log(q_sel) over one contour point.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_partial_qlogq(const double* __restrict__ qf,
+                                                       const double* __restrict__ qr, int use_qf,
+                                                       long n, double* __restrict__ part) {
+  __shared__ double red[NTA];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA) {
+    const double a = qf[i], b = qr[i];
+    if (a > 0.0 && b > 0.0) acc += a * b * log(use_qf ? a : b);
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// constant fill
+__global__ void __launch_bounds__(NTA) k_fill(double* __restrict__ x, long n, double v) {
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA) x[i] = v;
+}
+
+}  // namespace pscf

@@ -1,0 +1,1152 @@
+// Host side of libpscf_hip: plans, system state and the C ABI of
+// include/pscf_hip.h.  All compute runs in the gfx950 kernels of
+// kernels.hpp / kernels_aux.hpp.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pscf_hip.h"
+#include "kernels.hpp"
+#include "kernels_aux.hpp"
+
+using namespace pscf;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw Fail{std::string(#expr) + ": " + hipGetErrorString(e_)};                        \
+  } while (0)
+
+#define REQUIRE(cond, msg)                  \
+  do {                                                                                      \
+    if (!(cond)) throw Fail{msg};           \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_err.clear();
+    f();
+    return 0;
+  } catch (const Fail& e) {
+    g_err = e.msg;
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+}
+
+// ------------------------------------------------------------ device memory
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    alloc(count);
+    HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// --------------------------------------------------------------- twiddles
+// tw[m] = exp(-2 pi i m / N), m < N, computed with the angle reduced to the
+// first octant so that each entry is correctly rounded to ~1 ulp.
+std::mutex g_tw_mutex;
+std::map<int, double2*> g_tw;
+
+const double2* twiddles(int n) {
+  std::lock_guard<std::mutex> lk(g_tw_mutex);
+  auto it = g_tw.find(n);
+  if (it != g_tw.end()) return it->second;
+  std::vector<double2> h(n);
+  for (int m = 0; m < n; ++m) {
+    // angle = 2 pi m / n; use symmetric reduction for accuracy
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    h[m] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+  double2* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, sizeof(double2) * n));
+  HIPCHK(hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  g_tw[n] = d;
+  return d;
+}
+
+// ------------------------------------------------------- kernel dispatch
+using StepPlaneFn = void (*)(const StepTask*, int, int, const double2*, const double2*);
+using StepZFn = void (*)(const StepTask*, int, int, int, const double2*);
+using R2CFn = void (*)(const double*, long, double2*, long, const double2*, const double2*);
+using C2RFn = void (*)(const double2*, long, double*, long, double, const double2*, const double2*);
+using ZfftFn = void (*)(double2*, long, int, const double2*);
+using C2CFn = void (*)(double2*, long, const double2*, const double2*);
+
+struct PlaneSet {
+  StepPlaneFn a = nullptr, c = nullptr;
+  R2CFn r2c = nullptr;
+  C2RFn c2r = nullptr;
+  C2CFn c2c_f = nullptr, c2c_i = nullptr;
+};
+struct ZSet {
+  StepZFn z = nullptr;
+  ZfftFn half_f = nullptr, half_i = nullptr, full_f = nullptr, full_i = nullptr;
+};
+
+template <int A, int B>
+PlaneSet make_plane() {
+  PlaneSet s;
+  s.a = k_step_a<A, B>;
+  s.c = k_step_c<A, B>;
+  s.r2c = k_plane_r2c<A, B>;
+  s.c2r = k_plane_c2r<A, B>;
+  s.c2c_f = k_plane_c2c<A, B, -1>;
+  s.c2c_i = k_plane_c2c<A, B, +1>;
+  return s;
+}
+template <int A, int C>
+ZSet make_z() {
+  ZSet s;
+  s.z = k_step_z<A, C>;
+  s.half_f = k_zfft<A / 2 + 1, C, -1>;
+  s.half_i = k_zfft<A / 2 + 1, C, +1>;
+  s.full_f = k_zfft<A, C, -1>;
+  s.full_i = k_zfft<A, C, +1>;
+  return s;
+}
+
+// Grid sizes with compiled kernels: cubic/square grids of these sizes, and
+// 1D/2D grids whose unused dimensions are 1.
+#define PSCF_GRID_SIZES(X) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(96)
+
+const std::map<std::pair<int, int>, PlaneSet>& plane_table() {
+  static const std::map<std::pair<int, int>, PlaneSet> t = [] {
+    std::map<std::pair<int, int>, PlaneSet> m;
+#define PSCF_PL(n)                          \
+  m[{n, n}] = make_plane<n, n>();           \
+  m[{n, 1}] = make_plane<n, 1>();
+    PSCF_GRID_SIZES(PSCF_PL)
+#undef PSCF_PL
+    return m;
+  }();
+  return t;
+}
+const std::map<std::pair<int, int>, ZSet>& z_table() {
+  static const std::map<std::pair<int, int>, ZSet> t = [] {
+    std::map<std::pair<int, int>, ZSet> m;
+#define PSCF_Z(n)                 \
+  m[{n, n}] = make_z<n, n>();     \
+  m[{n, 1}] = make_z<n, 1>();
+    PSCF_GRID_SIZES(PSCF_Z)
+#undef PSCF_Z
+    return m;
+  }();
+  return t;
+}
+
+int grid_blocks(long n) {
+  long b = (n + NTA - 1) / NTA;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ------------------------------------------------------------- geometry
+struct Grid {
+  int n1 = 0, n2 = 0, n3 = 0, dim = 0, n1h = 0;
+  long m = 0, mk = 0;
+  const PlaneSet* pl = nullptr;
+  const ZSet* zs = nullptr;
+  const double2 *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;
+
+  void init(const int ng[3], int d) {
+    n1 = ng[0];
+    n2 = ng[1];
+    n3 = ng[2];
+    dim = d;
+    REQUIRE(n1 >= 2 && n2 >= 1 && n3 >= 1, "invalid grid dimensions");
+    REQUIRE(n1 % 2 == 0, "the first grid dimension must be even");
+    n1h = n1 / 2 + 1;
+    m = (long)n1 * n2 * n3;
+    mk = (long)n1h * n2 * n3;
+    auto ip = plane_table().find({n1, n2});
+    auto iz = z_table().find({n1, n3});
+    char buf[160];
+    snprintf(buf, sizeof buf, "grid %dx%dx%d has no compiled gfx950 kernels", n1, n2, n3);
+    REQUIRE(ip != plane_table().end() && iz != z_table().end(), buf);
+    pl = &ip->second;
+    zs = &iz->second;
+    tw1 = twiddles(n1);
+    tw2 = twiddles(n2);
+    tw3 = twiddles(n3);
+  }
+
+  // real (nb arrays, stride m) -> half spectra (stride mk)
+  void r2c(const double* in, double2* out, int nb, hipStream_t s) const {
+    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(NT), 0, s, in, m, out, mk, tw1, tw2);
+    if (n3 > 1) hipLaunchKernelGGL(zs->half_f, dim3(n2, nb), dim3(NT), 0, s, out, mk, n2, tw3);
+  }
+  // half spectra -> real, in place on `in` for the z pass
+  void c2r(double2* in, double* out, int nb, double scale, hipStream_t s) const {
+    if (n3 > 1) hipLaunchKernelGGL(zs->half_i, dim3(n2, nb), dim3(NT), 0, s, in, mk, n2, tw3);
+    hipLaunchKernelGGL(pl->c2r, dim3(n3, nb), dim3(NT), 0, s, in, mk, out, m, scale, tw1, tw2);
+  }
+  void c2c(double2* data, int sign, hipStream_t s) const {
+    hipLaunchKernelGGL(sign < 0 ? pl->c2c_f : pl->c2c_i, dim3(n3, 1), dim3(NT), 0, s, data, m, tw1,
+                       tw2);
+    if (n3 > 1)
+      hipLaunchKernelGGL(sign < 0 ? zs->full_f : zs->full_i, dim3(n2, 1), dim3(NT), 0, s, data, m,
+                         n2, tw3);
+  }
+};
+
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { HIPCHK(hipStreamCreate(&s)); }
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void sync() const { HIPCHK(hipStreamSynchronize(s)); }
+};
+
+// ------------------------------------------------------------ fft plan
+struct FftPlan {
+  Grid g;
+  Stream st;
+  DevBuf<double> r;
+  DevBuf<double2> k, c;
+};
+
+// -------------------------------------------------------- standalone step
+struct StepState {
+  Grid g;
+  Stream st;
+  DevBuf<double> fac;      // e1, e2 (m each), k1, k2 (mk each)
+  DevBuf<double> slab;     // 2 points
+  DevBuf<double> q1;
+  DevBuf<double2> abc;     // A, B, C
+  DevBuf<double> ksq, om;
+  DevBuf<StepTask> task;
+  bool have_propg = false;
+};
+
+// ---------------------------------------------------------------- system
+struct ChainDev {
+  int nblk = 0, ns = 0;
+  int bnd[MAXBLK + 1] = {0};
+  double ds[MAXBLK] = {0};
+  int mon[MAXBLK] = {0};   // 0-based
+  double length = 0;
+  DevBuf<double> qf, qr;   // (ns+1) * m
+  DevBuf<double> fac;      // per block: e1, e2 (m), k1, k2 (mk)
+};
+
+struct System {
+  Grid g;
+  Stream st;
+  // basis
+  int nstar = 0, nwave = 0;
+  DevBuf<int> star_of_wave, star_invert, wtab, kwave, sconj;
+  DevBuf<double2> coeff, scoef;
+  DevBuf<long> sidx;
+  std::vector<int> h_wave, h_wave_of_star;
+  std::vector<double2> h_coeff;
+  int gmin[3] = {0, 0, 0}, gext[3] = {1, 1, 1};
+  bool have_basis = false, have_cell = false;
+  // geometry
+  double gb[9] = {0};
+  DevBuf<double> ksq;
+  // chemistry
+  int nmon = 0, nchain = 0, nsolv = 0, ensemble = 0;
+  std::vector<double> kuhn, solv_size, chi;
+  std::vector<int> solv_mon;
+  std::vector<std::vector<std::pair<int, double>>> blocks;  // (monomer 0-based, length)
+  std::vector<std::unique_ptr<ChainDev>> chains;
+  bool have_chem = false, started = false;
+  // work
+  DevBuf<double> omega_in, rho_out, omega_grid, rho_grid, part, qdev, basis_tmp;
+  DevBuf<double2> kwork;
+  DevBuf<double2> abc;     // 3 * mk per task
+  DevBuf<double> q1;       // m per task
+  DevBuf<StepTask> tasks;
+  std::vector<double> h_Q, h_Qs;
+  bool have_density = false;
+  // stress work
+  DevBuf<double> sbuf;     // real chunk
+  DevBuf<double2> skbuf;   // spectra chunk
+  DevBuf<double> sbasis;   // star coefficients chunk
+  DevBuf<double> dgsq, sdots;
+};
+
+template <class T>
+T* from_handle(int64_t h, const char* what) {
+  REQUIRE(h != 0, std::string("null ") + what + " handle");
+  return reinterpret_cast<T*>(h);
+}
+
+// ----------------------------------------------------------- chain rule
+// src/chain_mod.f90:141-156
+void discretize(int nblk, const double* L, double ds0, int order, int* bgn, double* bds) {
+  int bgns = 1;
+  bgn[0] = bgns;
+  for (int i = 0; i < nblk; ++i) {
+    int iblk = (int)(L[i] / ds0 / 2.0 + 0.5);
+    if (iblk == 0) {
+      iblk = 1;
+      bds[i] = L[i] / 2.0;
+    } else {
+      bds[i] = L[i] / (double)iblk / 2.0;
+    }
+    if (order >= 0) {
+      bds[i] = bds[i] / std::pow(2.0, order);
+      iblk = iblk * (1 << order);
+    }
+    bgns = bgns + iblk * 2;
+    bgn[i + 1] = bgns;
+  }
+}
+
+// ------------------------------------------------------------ reductions
+// out_dev[0] = sum(x[0..n)) * scale (async)
+void dev_sum(const double* x, long n, double scale, double* part, double* out_dev, hipStream_t s) {
+  const int nb = grid_blocks(n) > 512 ? 512 : grid_blocks(n);
+  hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(NTA), 0, s, x, n, part);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, s, part, nb, scale, out_dev);
+}
+
+// --------------------------------------------------- contour sweeps
+// Advance the given sweep tasks through all their steps (4 launches per step).
+void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, hipStream_t s) {
+  for (int j = 0; j <= max_ns; ++j) {
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1, g.tw2);
+    if (j == max_ns) break;
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n2, 0, g.tw3);
+    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1,
+                       g.tw2);
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 1), dim3(NT), 0, s, d_tasks, j, g.n2, 1, g.tw3);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void build_basis_maps(System& S) {
+  // dense which_wave table over the bounding box of the wave list
+  const int dim = S.g.dim;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    lo[d] = 1 << 30;
+    hi[d] = -(1 << 30);
+  }
+  for (int w = 0; w < S.nwave; ++w)
+    for (int d = 0; d < dim; ++d) {
+      const int v = S.h_wave[(size_t)w * dim + d];
+      lo[d] = std::min(lo[d], v);
+      hi[d] = std::max(hi[d], v);
+    }
+  long tot = 1;
+  for (int d = 0; d < 3; ++d) {
+    S.gmin[d] = d < dim ? lo[d] : 0;
+    S.gext[d] = d < dim ? hi[d] - lo[d] + 1 : 1;
+    tot *= S.gext[d];
+  }
+  std::vector<int> wt(tot, 0);
+  auto box = [&](const int* G) -> long {
+    long off = 0, mul = 1;
+    for (int d = 0; d < dim; ++d) {
+      const int u = G[d] - S.gmin[d];
+      if (u < 0 || u >= S.gext[d]) return -1;
+      off += (long)u * mul;
+      mul *= S.gext[d];
+    }
+    return off;
+  };
+  for (int w = 0; w < S.nwave; ++w) wt[box(&S.h_wave[(size_t)w * dim])] = w + 1;
+  S.wtab.upload(wt.data(), wt.size(), S.st.s);
+  // per star: representative wave -> half-grid index (conj if kx > N1/2), coeff
+  std::vector<long> sidx(S.nstar);
+  std::vector<int> sconj(S.nstar);
+  std::vector<double2> scoef(S.nstar);
+  const int ng[3] = {S.g.n1, S.g.n2, S.g.n3};
+  for (int s = 0; s < S.nstar; ++s) {
+    int kbz[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) kbz[d] = S.h_wave_of_star[(size_t)s * dim + d];
+    const long o = box(kbz);
+    REQUIRE(o >= 0 && wt[o] > 0, "wave_of_star is not in the wave list");
+    scoef[s] = S.h_coeff[wt[o] - 1];
+    int kf[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) kf[d] = ((kbz[d] % ng[d]) + ng[d]) % ng[d];
+    sconj[s] = 0;
+    if (kf[0] > S.g.n1 / 2) {
+      for (int d = 0; d < dim; ++d) kf[d] = (((-kbz[d]) % ng[d]) + ng[d]) % ng[d];
+      sconj[s] = 1;
+    }
+    sidx[s] = kf[0] + (long)S.g.n1h * (kf[1] + (long)S.g.n2 * kf[2]);
+  }
+  S.sidx.upload(sidx.data(), sidx.size(), S.st.s);
+  S.sconj.upload(sconj.data(), sconj.size(), S.st.s);
+  S.scoef.upload(scoef.data(), scoef.size(), S.st.s);
+  S.st.sync();
+}
+
+void apply_cell(System& S) {
+  GeomArgs a;
+  a.n1 = S.g.n1;
+  a.n2 = S.g.n2;
+  a.n3 = S.g.n3;
+  a.dim = S.g.dim;
+  for (int i = 0; i < 9; ++i) a.gb[i] = S.gb[i];
+  for (int d = 0; d < 3; ++d) {
+    a.gmin[d] = S.gmin[d];
+    a.gext[d] = S.gext[d];
+  }
+  S.ksq.alloc(S.g.mk);
+  S.kwave.alloc(S.g.mk);
+  hipLaunchKernelGGL(k_g_to_bz, dim3(grid_blocks(S.g.mk)), dim3(NTA), 0, S.st.s, a, S.wtab.p, S.ksq.p,
+                     S.kwave.p);
+  HIPCHK(hipGetLastError());
+  S.have_cell = true;
+}
+
+// star coefficients (device, (nf, nstar)) -> real grids (nf arrays)
+void basis_to_grid(System& S, const double* d_basis, int nf, double* d_grid) {
+  const Grid& g = S.g;
+  S.kwork.alloc((size_t)std::max(nf, 2) * g.mk);
+  hipLaunchKernelGGL(k_basis_to_kgrid, dim3(grid_blocks(g.mk * nf)), dim3(NTA), 0, S.st.s, d_basis, nf,
+                     S.nstar, S.kwave.p, S.coeff.p, S.star_of_wave.p, S.star_invert.p, g.mk,
+                     S.kwork.p, 0);
+  g.c2r(S.kwork.p, d_grid, nf, 1.0, S.st.s);
+}
+
+// real grids (nf arrays, stride m) -> star coefficients (nf, nstar) * scale
+void grid_to_basis(System& S, const double* d_grid, int nf, double scale, double* d_basis) {
+  const Grid& g = S.g;
+  S.kwork.alloc((size_t)std::max(nf, 2) * g.mk);
+  g.r2c(d_grid, S.kwork.p, nf, S.st.s);
+  hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks((long)S.nstar * nf)), dim3(NTA), 0, S.st.s,
+                     S.kwork.p, g.mk, nf, S.nstar, S.sidx.p, S.sconj.p, S.scoef.p, S.star_invert.p,
+                     scale, d_basis, 0);
+}
+
+void startup(System& S, double chain_step) {
+  REQUIRE(S.have_chem, "set_chemistry must precede startup");
+  const Grid& g = S.g;
+  S.chains.clear();
+  for (int c = 0; c < S.nchain; ++c) {
+    auto ch = std::make_unique<ChainDev>();
+    const auto& bl = S.blocks[c];
+    ch->nblk = (int)bl.size();
+    REQUIRE(ch->nblk >= 1 && ch->nblk <= MAXBLK, "number of blocks per chain out of range");
+    std::vector<double> L(ch->nblk);
+    std::vector<int> bgn(ch->nblk + 1);
+    for (int b = 0; b < ch->nblk; ++b) {
+      L[b] = bl[b].second;
+      ch->mon[b] = bl[b].first;
+    }
+    discretize(ch->nblk, L.data(), chain_step, -1, bgn.data(), ch->ds);
+    for (int b = 0; b <= ch->nblk; ++b) ch->bnd[b] = bgn[b] - 1;
+    ch->ns = ch->bnd[ch->nblk];
+    ch->length = 0.0;
+    for (int b = 0; b < ch->nblk; ++b) ch->length = ch->length + L[b];  // chemistry_mod.f90:560-566
+    ch->qf.alloc((size_t)(ch->ns + 1) * g.m);
+    ch->qr.alloc((size_t)(ch->ns + 1) * g.m);
+    ch->fac.alloc((size_t)ch->nblk * (2 * g.m + 2 * g.mk));
+    S.chains.push_back(std::move(ch));
+  }
+  const int ntask = 2 * S.nchain;
+  S.abc.alloc((size_t)3 * g.mk * std::max(ntask, 1));
+  S.q1.alloc((size_t)g.m * std::max(ntask, 1));
+  S.tasks.alloc(std::max(ntask, 1));
+  S.omega_grid.alloc((size_t)S.nmon * g.m);
+  S.rho_grid.alloc((size_t)S.nmon * g.m);
+  S.part.alloc(1024);
+  S.qdev.alloc(S.nchain + S.nsolv + 1);
+  S.started = true;
+  S.have_density = false;
+}
+
+void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
+             double* mu_c, double* phi_s, double* mu_s) {
+  REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  const size_t nst = (size_t)S.nmon * S.nstar;
+  S.omega_in.upload(omega, nst, s);
+  // omega stars -> grid, no 1/M (src/scf_mod.fp.f90:182-185)
+  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p);
+  // sweeps of all chains, forward and backward together
+  std::vector<StepTask> ht(2 * S.nchain);
+  int max_ns = 0;
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    max_ns = std::max(max_ns, ch.ns);
+    for (int b = 0; b < ch.nblk; ++b) {
+      double* f = ch.fac.p + (size_t)b * (2 * g.m + 2 * g.mk);
+      hipLaunchKernelGGL(k_make_propg, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.ds[b],
+                         S.kuhn[ch.mon[b]], S.ksq.p, g.mk, S.omega_grid.p + (size_t)ch.mon[b] * g.m,
+                         g.m, f, f + g.m, f + 2 * g.m, f + 2 * g.m + g.mk);
+    }
+    for (int d = 0; d < 2; ++d) {
+      StepTask& t = ht[2 * c + d];
+      std::memset(&t, 0, sizeof t);
+      t.slab = d == 0 ? ch.qf.p : ch.qr.p;
+      t.ns = ch.ns;
+      t.dir = d == 0 ? 1 : -1;
+      t.init_one = 1;
+      t.nblk = ch.nblk;
+      for (int b = 0; b <= ch.nblk; ++b) t.bnd[b] = ch.bnd[b];
+      for (int b = 0; b < ch.nblk; ++b) {
+        double* f = ch.fac.p + (size_t)b * (2 * g.m + 2 * g.mk);
+        t.e1[b] = f;
+        t.e2[b] = f + g.m;
+        t.k1[b] = f + 2 * g.m;
+        t.k2[b] = f + 2 * g.m + g.mk;
+      }
+      const int ti = 2 * c + d;
+      t.A = S.abc.p + (size_t)(3 * ti) * g.mk;
+      t.B = t.A + g.mk;
+      t.C = t.B + g.mk;
+      t.q1 = S.q1.p + (size_t)ti * g.m;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(S.tasks.p, ht.data(), sizeof(StepTask) * ht.size(), hipMemcpyHostToDevice, s));
+  run_sweeps(g, S.tasks.p, (int)ht.size(), max_ns, s);
+  // Q = mean of qf at the chain end (src/scf_mod.fp.f90:359-360)
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    dev_sum(ch.qf.p + (size_t)ch.ns * g.m, g.m, 1.0 / (double)g.m, S.part.p, S.qdev.p + c, s);
+  }
+  HIPCHK(hipMemsetAsync(S.rho_grid.p, 0, sizeof(double) * S.nmon * g.m, s));
+  // solvents (src/scf_mod.fp.f90:193-200, 251-295)
+  for (int i = 0; i < S.nsolv; ++i) {
+    const double* w = S.omega_grid.p + (size_t)S.solv_mon[i] * g.m;
+    const int nb = 512;
+    hipLaunchKernelGGL(k_partial_sum_exp, dim3(nb), dim3(NTA), 0, s, w, S.solv_size[i], g.m, S.part.p);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, s, S.part.p, nb, 1.0 / (double)g.m,
+                       S.qdev.p + S.nchain + i);
+    hipLaunchKernelGGL(k_solvent_rho, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, w, S.solv_size[i],
+                       S.qdev.p + S.nchain + i, S.ensemble, phi_s ? phi_s[i] : 0.0,
+                       mu_s ? mu_s[i] : 0.0, g.m, S.rho_grid.p + (size_t)S.solv_mon[i] * g.m);
+  }
+  // Simpson rho per chain (src/scf_mod.fp.f90:362-424, 212-218)
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    SimpsonArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nblk = ch.nblk;
+    for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
+    for (int b = 0; b < ch.nblk; ++b) {
+      a.ds[b] = ch.ds[b];
+      a.monomer[b] = ch.mon[b];
+    }
+    a.chain_length = ch.length;
+    a.phi = phi_c ? phi_c[c] : 0.0;
+    a.mu = mu_c ? mu_c[c] : 0.0;
+    a.ensemble = S.ensemble;
+    hipLaunchKernelGGL(k_simpson, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a,
+                       S.qdev.p + c, S.rho_grid.p);
+  }
+  // rho grid -> stars, / M (src/scf_mod.fp.f90:221-225)
+  S.rho_out.alloc(nst);
+  grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p);
+  HIPCHK(hipGetLastError());
+  S.h_Q.assign(S.nchain + S.nsolv, 0.0);
+  HIPCHK(hipMemcpyAsync(S.h_Q.data(), S.qdev.p, sizeof(double) * (S.nchain + S.nsolv),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
+  S.st.sync();
+  for (int c = 0; c < S.nchain; ++c)
+    if (qout) qout[c] = S.h_Q[c];
+  for (int i = 0; i < S.nsolv; ++i)
+    if (qsolv) qsolv[i] = S.h_Q[S.nchain + i];
+  // grand ensemble: phi from mu (src/scf_mod.fp.f90:202-209, 768-820)
+  if (S.ensemble == 1) {
+    for (int c = 0; c < S.nchain; ++c)
+      if (phi_c && mu_c) phi_c[c] = S.h_Q[c] * std::exp(mu_c[c]);
+    for (int i = 0; i < S.nsolv; ++i)
+      if (phi_s && mu_s) phi_s[i] = S.h_Q[S.nchain + i] * std::exp(mu_s[i]);
+  }
+  S.have_density = true;
+}
+
+// scf_stress (src/scf_mod.fp.f90:451-550)
+void stress(System& S, int ncp, const double* dgsq, const double* phi_c, const double* mu_c,
+            double* out) {
+  REQUIRE(S.have_density, "scf_stress requires a previous density()");
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  S.dgsq.upload(dgsq, (size_t)S.nstar * ncp, s);
+  const double rnodes = (double)g.m;
+  const double normal = rnodes * rnodes * 3.0 * 6.0;        // :485-488
+  const int chunk = 32;
+  S.sbasis.alloc((size_t)2 * chunk * S.nstar);
+  S.skbuf.alloc((size_t)chunk * g.mk);
+  S.sdots.alloc((size_t)chunk * ncp);
+  for (int k = 0; k < ncp; ++k) out[k] = 0.0;
+  for (int sp = 0; sp < S.nchain; ++sp) {
+    ChainDev& ch = *S.chains[sp];
+    // dot products for every contour point 0..ns
+    std::vector<double> alld((size_t)(ch.ns + 1) * ncp);
+    for (int p0 = 0; p0 <= ch.ns; p0 += chunk) {
+      const int np = std::min(chunk, ch.ns + 1 - p0);
+      for (int d = 0; d < 2; ++d) {
+        const double* src = (d == 0 ? ch.qf.p : ch.qr.p) + (size_t)p0 * g.m;
+        g.r2c(src, S.skbuf.p, np, s);
+        hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks((long)S.nstar * np)), dim3(NTA), 0, s,
+                           S.skbuf.p, g.mk, np, S.nstar, S.sidx.p, S.sconj.p, S.scoef.p,
+                           S.star_invert.p, 1.0, S.sbasis.p + (size_t)d * chunk * S.nstar, 0);
+      }
+      hipLaunchKernelGGL(k_stress_dots, dim3(np), dim3(NTA), 0, s, S.sbasis.p,
+                         S.sbasis.p + (size_t)chunk * S.nstar, np, S.nstar, S.dgsq.p, ncp, S.sdots.p);
+      HIPCHK(hipMemcpyAsync(alld.data() + (size_t)p0 * ncp, S.sdots.p, sizeof(double) * np * ncp,
+                            hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipGetLastError());
+    S.st.sync();
+    std::vector<double> dQ(ncp, 0.0);
+    for (int b = 0; b < ch.nblk; ++b) {
+      const double bk = S.kuhn[ch.mon[b]];
+      const double ds0 = ch.ds[b];
+      const int ibgn = ch.bnd[b] + 1, iend = ch.bnd[b + 1] + 1;   // 1-based, as the reference
+      for (int i = ibgn; i <= iend; ++i) {
+        double ds = ds0;
+        if (i != ibgn && i != iend) ds = (i % 2 == 0) ? 4.0 * ds : 2.0 * ds;
+        for (int beta = 0; beta < ncp; ++beta) {
+          double inc = alld[(size_t)(i - 1) * ncp + beta];
+          inc = inc * (bk * bk) * ds / normal;
+          dQ[beta] = dQ[beta] - inc;
+        }
+      }
+    }
+    const double Q = S.h_Q[sp];
+    for (int beta = 0; beta < ncp; ++beta) {
+      if (S.ensemble == 0)
+        out[beta] = out[beta] - (dQ[beta] / Q) * phi_c[sp] / ch.length;
+      else
+        out[beta] = out[beta] - (dQ[beta] / Q) * std::exp(mu_c[sp]) * Q / ch.length;
+    }
+  }
+}
+
+}  // namespace
+
+// =====================================================================  C ABI
+extern "C" {
+
+const char* pscf_last_error(void) { return g_err.c_str(); }
+
+int pscf_device_count(int* count) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int pscf_chain_discretize(int nblk, const double* blk_length, double ds0, int order, int* block_bgn,
+                          double* block_ds) {
+  return guarded([&] { discretize(nblk, blk_length, ds0, order, block_bgn, block_ds); });
+}
+
+// ------------------------------------------------------------------ fft_mod
+int pscf_fft_plan_create(const int ngrid[3], int64_t* plan) {
+  return guarded([&] {
+    auto* p = new FftPlan;
+    try {
+      int dim = ngrid[2] > 1 ? 3 : (ngrid[1] > 1 ? 2 : 1);
+      p->g.init(ngrid, dim);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *plan = reinterpret_cast<int64_t>(p);
+  });
+}
+
+int pscf_fft_plan_destroy(int64_t plan) {
+  return guarded([&] { delete from_handle<FftPlan>(plan, "fft plan"); });
+}
+
+int pscf_fft_r2c(int64_t plan, const double* in, double* out) {
+  return guarded([&] {
+    FftPlan& P = *from_handle<FftPlan>(plan, "fft plan");
+    P.r.upload(in, P.g.m, P.st.s);
+    P.k.alloc(P.g.mk);
+    P.g.r2c(P.r.p, P.k.p, 1, P.st.s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, P.k.p, sizeof(double2) * P.g.mk, hipMemcpyDeviceToHost, P.st.s));
+    P.st.sync();
+  });
+}
+
+int pscf_fft_c2r(int64_t plan, const double* in, double* out) {
+  return guarded([&] {
+    FftPlan& P = *from_handle<FftPlan>(plan, "fft plan");
+    P.k.upload(reinterpret_cast<const double2*>(in), P.g.mk, P.st.s);
+    P.r.alloc(P.g.m);
+    P.g.c2r(P.k.p, P.r.p, 1, 1.0, P.st.s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, P.r.p, sizeof(double) * P.g.m, hipMemcpyDeviceToHost, P.st.s));
+    P.st.sync();
+  });
+}
+
+int pscf_fft_c2c(int64_t plan, int direction, const double* in, double* out) {
+  return guarded([&] {
+    FftPlan& P = *from_handle<FftPlan>(plan, "fft plan");
+    P.c.upload(reinterpret_cast<const double2*>(in), P.g.m, P.st.s);
+    P.g.c2c(P.c.p, direction == 1 ? -1 : +1, P.st.s);
+    HIPCHK(hipGetLastError());
+    std::vector<double2> h(P.g.m);
+    HIPCHK(hipMemcpyAsync(h.data(), P.c.p, sizeof(double2) * P.g.m, hipMemcpyDeviceToHost, P.st.s));
+    P.st.sync();
+    const double rM = (double)P.g.m;
+    for (long i = 0; i < P.g.m; ++i) {
+      double2 v = h[i];
+      if (direction == +1) v = make_double2(v.x / rM, v.y / rM);   // src/fft3_mod.f90:156
+      out[2 * i] = v.x;
+      out[2 * i + 1] = v.y;
+    }
+  });
+}
+
+int pscf_fft_r2c_dev(int64_t plan, const double* in, double* out, int nbatch) {
+  return guarded([&] {
+    FftPlan& P = *from_handle<FftPlan>(plan, "fft plan");
+    P.g.r2c(in, reinterpret_cast<double2*>(out), nbatch, P.st.s);
+    HIPCHK(hipGetLastError());
+    P.st.sync();
+  });
+}
+
+int pscf_fft_c2r_dev(int64_t plan, const double* in, double* out, int nbatch) {
+  return guarded([&] {
+    FftPlan& P = *from_handle<FftPlan>(plan, "fft plan");
+    // the z pass works in place: copy the spectra first so `in` is preserved
+    P.k.alloc((size_t)P.g.mk * nbatch);
+    HIPCHK(hipMemcpyAsync(P.k.p, in, sizeof(double2) * P.g.mk * nbatch, hipMemcpyDeviceToDevice,
+                          P.st.s));
+    P.g.c2r(P.k.p, out, nbatch, 1.0, P.st.s);
+    HIPCHK(hipGetLastError());
+    P.st.sync();
+  });
+}
+
+// ----------------------------------------------------------------- step_mod
+int pscf_step_create(const int ngrid[3], int64_t* step) {
+  return guarded([&] {
+    auto* p = new StepState;
+    try {
+      int dim = ngrid[2] > 1 ? 3 : (ngrid[1] > 1 ? 2 : 1);
+      p->g.init(ngrid, dim);
+      const Grid& g = p->g;
+      p->fac.alloc(2 * g.m + 2 * g.mk);
+      p->slab.alloc(2 * g.m);
+      p->q1.alloc(g.m);
+      p->abc.alloc(3 * g.mk);
+      p->task.alloc(1);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *step = reinterpret_cast<int64_t>(p);
+  });
+}
+
+int pscf_step_destroy(int64_t step) {
+  return guarded([&] { delete from_handle<StepState>(step, "step"); });
+}
+
+int pscf_step_make_propg(int64_t step, double ds, double b, const double* ksq, const double* omega) {
+  return guarded([&] {
+    StepState& S = *from_handle<StepState>(step, "step");
+    const Grid& g = S.g;
+    S.ksq.upload(ksq, g.mk, S.st.s);
+    S.om.upload(omega, g.m, S.st.s);
+    double* f = S.fac.p;
+    hipLaunchKernelGGL(k_make_propg, dim3(grid_blocks(g.m)), dim3(NTA), 0, S.st.s, ds, b, S.ksq.p,
+                       g.mk, S.om.p, g.m, f, f + g.m, f + 2 * g.m, f + 2 * g.m + g.mk);
+    HIPCHK(hipGetLastError());
+    S.st.sync();
+    S.have_propg = true;
+  });
+}
+
+int pscf_step_exp(int64_t step, const double* q_in, double* q_out) {
+  return guarded([&] {
+    StepState& S = *from_handle<StepState>(step, "step");
+    REQUIRE(S.have_propg, "step_exp requires make_propg");
+    const Grid& g = S.g;
+    HIPCHK(hipMemcpyAsync(S.slab.p, q_in, sizeof(double) * g.m, hipMemcpyHostToDevice, S.st.s));
+    StepTask t;
+    std::memset(&t, 0, sizeof t);
+    t.slab = S.slab.p;
+    t.ns = 1;
+    t.dir = 1;
+    t.init_one = 0;
+    t.nblk = 1;
+    t.bnd[0] = 0;
+    t.bnd[1] = 1;
+    double* f = S.fac.p;
+    t.e1[0] = f;
+    t.e2[0] = f + g.m;
+    t.k1[0] = f + 2 * g.m;
+    t.k2[0] = f + 2 * g.m + g.mk;
+    t.A = S.abc.p;
+    t.B = S.abc.p + g.mk;
+    t.C = S.abc.p + 2 * g.mk;
+    t.q1 = S.q1.p;
+    HIPCHK(hipMemcpyAsync(S.task.p, &t, sizeof t, hipMemcpyHostToDevice, S.st.s));
+    run_sweeps(g, S.task.p, 1, 1, S.st.s);
+    HIPCHK(hipMemcpyAsync(q_out, S.slab.p + g.m, sizeof(double) * g.m, hipMemcpyDeviceToHost,
+                          S.st.s));
+    S.st.sync();
+  });
+}
+
+// ------------------------------------------------------------------ scf_mod
+int pscf_system_create(const int ngrid[3], int dim, int64_t* sys) {
+  return guarded([&] {
+    auto* p = new System;
+    try {
+      p->g.init(ngrid, dim);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *sys = reinterpret_cast<int64_t>(p);
+  });
+}
+
+int pscf_system_destroy(int64_t sys) {
+  return guarded([&] { delete from_handle<System>(sys, "system"); });
+}
+
+int pscf_system_set_basis(int64_t sys, int n_wave, int n_star, const int* wave, const double* coeff,
+                          const int* star_of_wave, const int* star_invert, const int* wave_of_star) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(n_wave > 0 && n_star > 0, "empty basis");
+    const int dim = S.g.dim;
+    S.nwave = n_wave;
+    S.nstar = n_star;
+    S.h_wave.assign(wave, wave + (size_t)dim * n_wave);
+    S.h_wave_of_star.assign(wave_of_star, wave_of_star + (size_t)dim * n_star);
+    S.h_coeff.resize(n_wave);
+    for (int i = 0; i < n_wave; ++i) S.h_coeff[i] = make_double2(coeff[2 * i], coeff[2 * i + 1]);
+    for (int i = 0; i < n_star; ++i)
+      REQUIRE(star_invert[i] >= -1 && star_invert[i] <= 1,
+              "Illegal star_invert value in basis_to_kgrid.");
+    for (int i = 0; i < n_star; ++i) {
+      if (star_invert[i] == 1)
+        REQUIRE(i + 1 < n_star && star_invert[i + 1] == -1, "star_invert(i)=1 without -1 partner");
+      if (star_invert[i] == -1) REQUIRE(i > 0 && star_invert[i - 1] == 1, "star_invert(i)=-1 without +1 partner");
+    }
+    for (int i = 0; i < n_wave; ++i)
+      REQUIRE(star_of_wave[i] >= 1 && star_of_wave[i] <= n_star, "star_of_wave out of range");
+    S.coeff.upload(S.h_coeff.data(), n_wave, S.st.s);
+    S.star_of_wave.upload(star_of_wave, n_wave, S.st.s);
+    S.star_invert.upload(star_invert, n_star, S.st.s);
+    build_basis_maps(S);
+    S.have_basis = true;
+    if (S.have_cell) {
+      apply_cell(S);
+      S.st.sync();
+    }
+  });
+}
+
+int pscf_system_set_cell(int64_t sys, const double* G_basis) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_basis, "set_basis must precede set_cell");
+    const int dim = S.g.dim;
+    for (int i = 0; i < 9; ++i) S.gb[i] = 0.0;
+    // Fortran G_basis(dim,dim), element (i,j) at G_basis[i + dim*j]
+    for (int i = 0; i < dim; ++i)
+      for (int jj = 0; jj < dim; ++jj) S.gb[i * 3 + jj] = G_basis[i + dim * jj];
+    apply_cell(S);
+    S.st.sync();
+  });
+}
+
+int pscf_system_get_ksq(int64_t sys, double* ksq) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_cell, "no cell");
+    HIPCHK(hipMemcpy(ksq, S.ksq.p, sizeof(double) * S.g.mk, hipMemcpyDeviceToHost));
+  });
+}
+
+int pscf_system_set_chemistry(int64_t sys, int n_monomer, const double* kuhn, int n_chain,
+                              const int* n_block, int n_blk_max, const int* block_monomer,
+                              const double* block_length, int n_solvent,
+                              const int* solvent_monomer, const double* solvent_size,
+                              int ensemble) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(n_monomer >= 1 && n_chain >= 0 && n_solvent >= 0, "invalid chemistry sizes");
+    REQUIRE(ensemble == 0 || ensemble == 1, "Invalid ensemble");
+    S.nmon = n_monomer;
+    S.nchain = n_chain;
+    S.nsolv = n_solvent;
+    S.ensemble = ensemble;
+    S.kuhn.assign(kuhn, kuhn + n_monomer);
+    S.blocks.assign(n_chain, {});
+    for (int c = 0; c < n_chain; ++c) {
+      REQUIRE(n_block[c] >= 1 && n_block[c] <= n_blk_max && n_block[c] <= MAXBLK,
+              "N_block out of range");
+      for (int b = 0; b < n_block[c]; ++b) {
+        const int mon = block_monomer[b + (size_t)n_blk_max * c];
+        REQUIRE(mon >= 1 && mon <= n_monomer, "block_monomer out of range");
+        S.blocks[c].push_back({mon - 1, block_length[b + (size_t)n_blk_max * c]});
+      }
+    }
+    S.solv_mon.clear();
+    S.solv_size.clear();
+    for (int i = 0; i < n_solvent; ++i) {
+      REQUIRE(solvent_monomer[i] >= 1 && solvent_monomer[i] <= n_monomer,
+              "solvent_monomer out of range");
+      S.solv_mon.push_back(solvent_monomer[i] - 1);
+      S.solv_size.push_back(solvent_size[i]);
+    }
+    S.have_chem = true;
+    S.started = false;
+  });
+}
+
+int pscf_system_set_chi(int64_t sys, const double* chi) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_chem, "set_chemistry must precede set_chi");
+    S.chi.assign(chi, chi + (size_t)S.nmon * S.nmon);
+  });
+}
+
+// chain length = sum of block lengths (src/chemistry_mod.f90:560-566)
+static double chain_len(const System& S, int c) {
+  double L = 0.0;
+  for (const auto& b : S.blocks[c]) L = L + b.second;
+  return L;
+}
+
+int pscf_mu_phi_chain(int64_t sys, double* mu, double* phi, const double* q) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    for (int i = 0; i < S.nchain; ++i) {
+      if (S.ensemble == 0) mu[i] = std::log(phi[i] / q[i]);
+      else phi[i] = q[i] * std::exp(mu[i]);
+    }
+  });
+}
+
+int pscf_mu_phi_solvent(int64_t sys, double* mu, double* phi, const double* q) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    for (int i = 0; i < S.nsolv; ++i) {
+      if (S.ensemble == 0) mu[i] = std::log(phi[i] / q[i]);
+      else phi[i] = q[i] * std::exp(mu[i]);
+    }
+  });
+}
+
+int pscf_free_energy(int64_t sys, const double* rho, const double* omega, const double* phi_chain,
+                     const double* mu_chain, const double* phi_solvent, const double* mu_solvent,
+                     double* f_helmholtz, double* pressure) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.chi.size() == (size_t)S.nmon * S.nmon, "chi not set");
+    const int nm = S.nmon;
+    double f = 0.0;
+    for (int i = 0; i < S.nchain; ++i)
+      if (phi_chain[i] > 1.0e-8f) f = f + phi_chain[i] * (mu_chain[i] - 1.0) / chain_len(S, i);
+    for (int i = 0; i < S.nsolv; ++i)
+      if (phi_solvent[i] > 1.0e-8f) f = f + phi_solvent[i] * (mu_solvent[i] - 1.0) / S.solv_size[i];
+    for (int i = 0; i < S.nstar; ++i)
+      for (int a = 0; a < nm; ++a) {
+        for (int b = a + 1; b < nm; ++b)
+          f = f + rho[a + (size_t)nm * i] * S.chi[a + (size_t)nm * b] * rho[b + (size_t)nm * i];
+        f = f - omega[a + (size_t)nm * i] * rho[a + (size_t)nm * i];
+      }
+    *f_helmholtz = f;
+    if (pressure) {
+      double p = -f;
+      for (int i = 0; i < S.nchain; ++i)
+        if (phi_chain[i] > 1.0e-8f) p = p + mu_chain[i] * phi_chain[i] / chain_len(S, i);
+      for (int i = 0; i < S.nsolv; ++i)
+        if (phi_solvent[i] > 1.0e-8f) p = p + mu_solvent[i] * phi_solvent[i] / S.solv_size[i];
+      *pressure = p;
+    }
+  });
+}
+
+int pscf_free_energy_fh(int64_t sys, const double* phi_chain, const double* phi_solvent, double* fo) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.chi.size() == (size_t)S.nmon * S.nmon, "chi not set");
+    double f = 0.0;
+    std::vector<double> rho(S.nmon, 0.0);
+    for (int i = 0; i < S.nchain; ++i) {
+      const double L = chain_len(S, i);
+      if (phi_chain[i] > 1.0e-8f) f = f + (phi_chain[i] / L) * (std::log(phi_chain[i]) - 1);
+      for (const auto& b : S.blocks[i]) rho[b.first] = rho[b.first] + phi_chain[i] * b.second / L;
+    }
+    if (phi_solvent)
+      for (int i = 0; i < S.nsolv; ++i) {
+        if (phi_solvent[i] > 1.0e-8f)
+          f = f + (phi_solvent[i] / S.solv_size[i]) * (std::log(phi_solvent[i]) - 1);
+        rho[S.solv_mon[i]] = rho[S.solv_mon[i]] + phi_solvent[i];
+      }
+    for (int i = 0; i < S.nmon - 1; ++i)
+      for (int j = i + 1; j < S.nmon; ++j) f = f + S.chi[i + (size_t)S.nmon * j] * rho[i] * rho[j];
+    *fo = f;
+  });
+}
+
+int pscf_set_omega_uniform(int64_t sys, double* omega, const double* phi_chain,
+                           const double* phi_solvent) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.chi.size() == (size_t)S.nmon * S.nmon, "chi not set");
+    std::vector<double> phi_mon(S.nmon, 0.0);
+    for (int i = 0; i < S.nchain; ++i) {
+      const double L = chain_len(S, i);
+      for (const auto& b : S.blocks[i])
+        phi_mon[b.first] = phi_mon[b.first] + phi_chain[i] * b.second / L;
+    }
+    for (int i = 0; i < S.nsolv; ++i) phi_mon[S.solv_mon[i]] = phi_mon[S.solv_mon[i]] + phi_solvent[i];
+    for (int a = 0; a < S.nmon; ++a) {
+      omega[a] = 0.0;   // omega(alpha, 1): first star, column-major (N_monomer, N_star)
+      for (int b = 0; b < S.nmon; ++b) omega[a] = omega[a] + S.chi[a + (size_t)S.nmon * b] * phi_mon[b];
+    }
+  });
+}
+
+int pscf_system_startup(int64_t sys, double chain_step) {
+  return guarded([&] { startup(*from_handle<System>(sys, "system"), chain_step); });
+}
+
+int pscf_density(int64_t sys, const double* omega, double* rho, double* qout, double* q_solvent,
+                 double* phi_chain, double* mu_chain, double* phi_solvent, double* mu_solvent) {
+  return guarded([&] {
+    density(*from_handle<System>(sys, "system"), omega, rho, qout, q_solvent, phi_chain, mu_chain,
+            phi_solvent, mu_solvent);
+  });
+}
+
+int pscf_scf_stress(int64_t sys, int ncp, const double* dGsq, const double* phi_chain,
+                    const double* mu_chain, double* out) {
+  return guarded([&] {
+    stress(*from_handle<System>(sys, "system"), ncp, dGsq, phi_chain, mu_chain, out);
+  });
+}
+
+int pscf_divide_energy_terms(int64_t sys, double* out) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_density, "divide_energy requires a previous density()");
+    const Grid& g = S.g;
+    for (int c = 0; c < S.nchain; ++c) {
+      ChainDev& ch = *S.chains[c];
+      const int nh = ch.bnd[1], nt = ch.bnd[ch.nblk - 1];
+      for (int w = 0; w < 2; ++w) {
+        const int pt = w == 0 ? nh : nt;
+        const int nb = 512;
+        hipLaunchKernelGGL(k_partial_qlogq, dim3(nb), dim3(NTA), 0, S.st.s, ch.qf.p + (size_t)pt * g.m,
+                           ch.qr.p + (size_t)pt * g.m, w == 0 ? 1 : 0, g.m, S.part.p);
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, S.st.s, S.part.p, nb,
+                           1.0 / (S.h_Q[c] * (double)g.m), S.qdev.p + S.nchain + S.nsolv);
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, S.qdev.p + S.nchain + S.nsolv, sizeof(double),
+                              hipMemcpyDeviceToHost, S.st.s));
+        S.st.sync();
+        out[2 * c + w] = v;
+      }
+    }
+  });
+}
+
+int pscf_system_get_grids(int64_t sys, double* omega_grid, double* rho_grid) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_density, "no density computed yet");
+    const size_t n = (size_t)S.nmon * S.g.m;
+    if (omega_grid) HIPCHK(hipMemcpy(omega_grid, S.omega_grid.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (rho_grid) HIPCHK(hipMemcpy(rho_grid, S.rho_grid.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int pscf_system_get_q(int64_t sys, int chain, int dir, int s, double* q) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_density, "no density computed yet");
+    REQUIRE(chain >= 0 && chain < S.nchain, "chain index out of range");
+    ChainDev& ch = *S.chains[chain];
+    REQUIRE(s >= 0 && s <= ch.ns, "contour index out of range");
+    const double* src = (dir > 0 ? ch.qf.p : ch.qr.p) + (size_t)s * S.g.m;
+    HIPCHK(hipMemcpy(q, src, sizeof(double) * S.g.m, hipMemcpyDeviceToHost));
+  });
+}
+
+// ----------------------------------------------------------- grid_basis_mod
+int pscf_basis_to_kgrid(int64_t sys, const double* basis, double* kgrid, int no_parity) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_basis && S.have_cell, "basis/cell not set");
+    S.basis_tmp.upload(basis, S.nstar, S.st.s);
+    S.kwork.alloc((size_t)2 * S.g.mk);
+    hipLaunchKernelGGL(k_basis_to_kgrid, dim3(grid_blocks(S.g.mk)), dim3(NTA), 0, S.st.s,
+                       S.basis_tmp.p, 1, S.nstar, S.kwave.p, S.coeff.p, S.star_of_wave.p,
+                       S.star_invert.p, S.g.mk, S.kwork.p, no_parity);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(kgrid, S.kwork.p, sizeof(double2) * S.g.mk, hipMemcpyDeviceToHost, S.st.s));
+    S.st.sync();
+  });
+}
+
+int pscf_kgrid_to_basis(int64_t sys, const double* kgrid, double* basis, int no_parity) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_basis, "basis not set");
+    S.kwork.alloc((size_t)2 * S.g.mk);
+    HIPCHK(hipMemcpyAsync(S.kwork.p, kgrid, sizeof(double2) * S.g.mk, hipMemcpyHostToDevice, S.st.s));
+    S.basis_tmp.alloc(S.nstar);
+    hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks(S.nstar)), dim3(NTA), 0, S.st.s, S.kwork.p,
+                       S.g.mk, 1, S.nstar, S.sidx.p, S.sconj.p, S.scoef.p, S.star_invert.p, 1.0,
+                       S.basis_tmp.p, no_parity);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(basis, S.basis_tmp.p, sizeof(double) * S.nstar, hipMemcpyDeviceToHost,
+                          S.st.s));
+    S.st.sync();
+  });
+}
+
+}  // extern "C"

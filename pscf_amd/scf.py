@@ -1,0 +1,272 @@
+"""Host-side mirror of the reference's fft_mod / step_mod / chain_mod /
+scf_mod / grid_basis_mod interfaces over libpscf_hip.
+
+Names, argument meaning and error behaviour follow the reference modules
+(src/fft3_mod.f90, src/step_mod.f90, src/chain_mod.f90, src/scf_mod.fp.f90,
+src/grid_basis_mod.f90); every array crosses the C ABI in the reference's
+Fortran layout.  numpy conventions used here:
+  * grid fields: shape (N3, N2, N1), C order (== Fortran (N1,N2,N3), x fastest)
+  * half spectra: shape (N3, N2, N1//2+1), complex128
+  * star fields:  shape (N_monomer, N_star)
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import dptr, iptr, lib as _default_lib
+
+c_i64 = ctypes.c_int64
+
+
+def _grid3(ngrid):
+    g = [int(v) for v in ngrid] + [1] * (3 - len(ngrid))
+    return (ctypes.c_int * 3)(*g[:3]), g[:3]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _fortran2(a):
+    """C-contiguous buffer holding a 2D array in Fortran (column-major) order."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+
+
+class FFTPlan:
+    """fft_plan + fft/ifft/fftc (src/fft3_mod.f90:44-157)."""
+
+    def __init__(self, ngrid, lib=None):
+        self.lib = lib or _default_lib()
+        arr, self.ngrid = _grid3(ngrid)
+        self.shape = (self.ngrid[2], self.ngrid[1], self.ngrid[0])
+        self.kshape = (self.ngrid[2], self.ngrid[1], self.ngrid[0] // 2 + 1)
+        h = c_i64(0)
+        self.lib.call("pscf_fft_plan_create", arr, ctypes.byref(h))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", 0):
+            self.lib.dll.pscf_fft_plan_destroy(self.h)
+            self.h = 0
+
+    def fft(self, r):
+        """Unnormalised r2c, sign -1."""
+        r = _f64(r).reshape(self.shape)
+        out = np.empty(self.kshape, dtype=np.complex128)
+        self.lib.call("pscf_fft_r2c", self.h, dptr(r), dptr(out))
+        return out
+
+    def ifft(self, k):
+        """Unnormalised c2r, sign +1."""
+        k = np.ascontiguousarray(k, dtype=np.complex128).reshape(self.kshape)
+        out = np.empty(self.shape)
+        self.lib.call("pscf_fft_c2r", self.h, dptr(k), dptr(out))
+        return out
+
+    def fftc(self, direction, c):
+        """Full complex transform: direction +1 -> sign -1 and 1/M; -1 -> sign +1."""
+        c = np.ascontiguousarray(c, dtype=np.complex128).reshape(self.shape)
+        out = np.empty(self.shape, dtype=np.complex128)
+        self.lib.call("pscf_fft_c2c", self.h, int(direction), dptr(c), dptr(out))
+        return out
+
+
+class Step:
+    """init_step / make_propg / step_exp (src/step_mod.f90:60-176)."""
+
+    def __init__(self, ngrid, lib=None):
+        self.lib = lib or _default_lib()
+        arr, self.ngrid = _grid3(ngrid)
+        self.shape = (self.ngrid[2], self.ngrid[1], self.ngrid[0])
+        self.kshape = (self.ngrid[2], self.ngrid[1], self.ngrid[0] // 2 + 1)
+        h = c_i64(0)
+        self.lib.call("pscf_step_create", arr, ctypes.byref(h))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", 0):
+            self.lib.dll.pscf_step_destroy(self.h)
+            self.h = 0
+
+    def make_propg(self, ds, b, ksq, omega):
+        ksq = _f64(ksq).reshape(self.kshape)
+        omega = _f64(omega).reshape(self.shape)
+        self.lib.call("pscf_step_make_propg", self.h, float(ds), float(b), dptr(ksq), dptr(omega))
+
+    def step_exp(self, q_in):
+        q_in = _f64(q_in).reshape(self.shape)
+        out = np.empty(self.shape)
+        self.lib.call("pscf_step_exp", self.h, dptr(q_in), dptr(out))
+        return out
+
+
+def chain_discretize(blk_length, ds0, order=None, lib=None):
+    """make_chain_grid's block_bgn (1-based) and block_ds (src/chain_mod.f90:141-156)."""
+    lib = lib or _default_lib()
+    L = _f64(blk_length)
+    n = len(L)
+    bgn = np.zeros(n + 1, dtype=np.int32)
+    bds = np.zeros(n)
+    lib.call("pscf_chain_discretize", n, dptr(L), float(ds0), -1 if order is None else int(order),
+             iptr(bgn), dptr(bds))
+    return bgn, bds
+
+
+class Basis:
+    """Star-basis tables of basis_mod (src/basis_mod.f90:60-84).
+
+    wave (N_wave, dim), coeff (N_wave,) complex, star_of_wave (N_wave,),
+    star_invert (N_star,), wave_of_star (N_star, dim); indices 1-based."""
+
+    def __init__(self, wave, coeff, star_of_wave, star_invert, wave_of_star):
+        self.wave = np.ascontiguousarray(wave, dtype=np.int32)
+        self.coeff = np.ascontiguousarray(coeff, dtype=np.complex128)
+        self.star_of_wave = np.ascontiguousarray(star_of_wave, dtype=np.int32)
+        self.star_invert = np.ascontiguousarray(star_invert, dtype=np.int32)
+        self.wave_of_star = np.ascontiguousarray(wave_of_star, dtype=np.int32)
+        self.N_wave = len(self.coeff)
+        self.N_star = len(self.star_invert)
+
+    @classmethod
+    def from_tables(cls, d):
+        return cls(d["wave"], d["coeff"], d["star_of_wave"], d["star_invert"], d["wave_of_star"])
+
+
+class SCFSystem:
+    """One SCFT system on the device: the state scf_mod keeps in module
+    globals (src/scf_mod.fp.f90:53-55), driven through density / scf_stress /
+    free_energy with the reference's argument meaning."""
+
+    def __init__(self, system, basis, G_basis, lib=None):
+        self.lib = lib or _default_lib()
+        self.system = system
+        arr, self.ngrid = _grid3(system.ngrid3)
+        self.shape = (self.ngrid[2], self.ngrid[1], self.ngrid[0])
+        self.kshape = (self.ngrid[2], self.ngrid[1], self.ngrid[0] // 2 + 1)
+        h = c_i64(0)
+        self.lib.call("pscf_system_create", arr, int(system.dim), ctypes.byref(h))
+        self.h = h.value
+        self.basis = basis
+        self.N_star = basis.N_star
+        self.lib.call("pscf_system_set_basis", self.h, basis.N_wave, basis.N_star,
+                      iptr(basis.wave), dptr(basis.coeff), iptr(basis.star_of_wave),
+                      iptr(basis.star_invert), iptr(basis.wave_of_star))
+        self.set_cell(G_basis)
+        self.set_chemistry(system)
+        self.startup(system.chain_step)
+
+    def __del__(self):
+        if getattr(self, "h", 0):
+            self.lib.dll.pscf_system_destroy(self.h)
+            self.h = 0
+
+    # -------------------------------------------------------------- setup
+    def set_cell(self, G_basis):
+        self.G_basis = np.array(G_basis, dtype=np.float64)
+        self.lib.call("pscf_system_set_cell", self.h, dptr(_fortran2(self.G_basis)))
+
+    def ksq(self):
+        out = np.empty(self.kshape)
+        self.lib.call("pscf_system_get_ksq", self.h, dptr(out))
+        return out
+
+    def set_chemistry(self, s):
+        nblk = [len(c) for c in s.chains]
+        nmax = max(nblk) if nblk else 1
+        bm = np.zeros((max(s.N_chain, 1), nmax), dtype=np.int32)
+        bl = np.zeros((max(s.N_chain, 1), nmax))
+        for i, c in enumerate(s.chains):
+            for j, (m, L) in enumerate(c):
+                bm[i, j] = m
+                bl[i, j] = L
+        sm = np.array([m for m, _ in s.solvents] or [0], dtype=np.int32)
+        ss = np.array([z for _, z in s.solvents] or [0.0])
+        self.lib.call("pscf_system_set_chemistry", self.h, s.N_monomer, dptr(_f64(s.kuhn)),
+                      s.N_chain, iptr(np.array(nblk or [1], dtype=np.int32)), nmax, iptr(bm),
+                      dptr(bl), len(s.solvents), iptr(sm), dptr(ss), int(s.ensemble))
+        self.lib.call("pscf_system_set_chi", self.h, dptr(_fortran2(s.chi)))
+        self.phi_chain = _f64(s.phi_chain).copy()
+        self.mu_chain = _f64(s.mu_chain).copy()
+        self.phi_solvent = _f64(list(s.phi_solvent) or [0.0]).copy()
+        self.mu_solvent = np.zeros_like(self.phi_solvent)
+
+    def startup(self, chain_step):
+        """density_startup (src/scf_mod.fp.f90:82-117)."""
+        self.lib.call("pscf_system_startup", self.h, float(chain_step))
+
+    # ------------------------------------------------------------ hot path
+    def density(self, omega):
+        """density(N, omega, rho, qout, q_solvent) (src/scf_mod.fp.f90:144-229).
+        Returns (rho (N_monomer, N_star), q (N_chain,), q_solvent (N_solvent,))."""
+        s = self.system
+        om = _fortran2(omega)
+        rho = np.empty_like(om)
+        q = np.zeros(max(s.N_chain, 1))
+        qs = np.zeros(max(len(s.solvents), 1))
+        self.lib.call("pscf_density", self.h, dptr(om), dptr(rho), dptr(q), dptr(qs),
+                      dptr(self.phi_chain), dptr(self.mu_chain), dptr(self.phi_solvent),
+                      dptr(self.mu_solvent))
+        return rho.T.copy(), q[: s.N_chain].copy(), qs[: len(s.solvents)].copy()
+
+    def scf_stress(self, dGsq):
+        """scf_stress(N, size_dGsq, dGsq) (src/scf_mod.fp.f90:451-550)."""
+        dG = np.asarray(dGsq, dtype=np.float64).reshape(self.N_star, -1)
+        ncp = dG.shape[1]
+        out = np.zeros(ncp)
+        self.lib.call("pscf_scf_stress", self.h, ncp, dptr(_fortran2(dG)), dptr(self.phi_chain),
+                      dptr(self.mu_chain), dptr(out))
+        return out
+
+    # ---------------------------------------------------- thermodynamics
+    def mu_phi_chain(self, q):
+        self.lib.call("pscf_mu_phi_chain", self.h, dptr(self.mu_chain), dptr(self.phi_chain),
+                      dptr(_f64(q)))
+
+    def mu_phi_solvent(self, qs):
+        if len(self.system.solvents):
+            self.lib.call("pscf_mu_phi_solvent", self.h, dptr(self.mu_solvent),
+                          dptr(self.phi_solvent), dptr(_f64(qs)))
+
+    def free_energy(self, rho, omega):
+        f = ctypes.c_double(0.0)
+        p = ctypes.c_double(0.0)
+        self.lib.call("pscf_free_energy", self.h, dptr(_fortran2(rho)), dptr(_fortran2(omega)),
+                      dptr(self.phi_chain), dptr(self.mu_chain), dptr(self.phi_solvent),
+                      dptr(self.mu_solvent), ctypes.byref(f), ctypes.byref(p))
+        return f.value, p.value
+
+    def free_energy_FH(self):
+        f = ctypes.c_double(0.0)
+        self.lib.call("pscf_free_energy_fh", self.h, dptr(self.phi_chain), dptr(self.phi_solvent),
+                      ctypes.byref(f))
+        return f.value
+
+    def set_omega_uniform(self, omega):
+        om = _fortran2(omega)
+        self.lib.call("pscf_set_omega_uniform", self.h, dptr(om), dptr(self.phi_chain),
+                      dptr(self.phi_solvent))
+        return om.T.copy()
+
+    # ------------------------------------------------------- inspection
+    def grids(self):
+        nm = self.system.N_monomer
+        og = np.empty((nm,) + self.shape)
+        rg = np.empty((nm,) + self.shape)
+        self.lib.call("pscf_system_get_grids", self.h, dptr(og), dptr(rg))
+        return og, rg
+
+    def q(self, chain, direction, s):
+        out = np.empty(self.shape)
+        self.lib.call("pscf_system_get_q", self.h, int(chain), int(direction), int(s), dptr(out))
+        return out
+
+    def basis_to_kgrid(self, basis, no_parity=False):
+        out = np.empty(self.kshape, dtype=np.complex128)
+        self.lib.call("pscf_basis_to_kgrid", self.h, dptr(_f64(basis)), dptr(out), int(no_parity))
+        return out
+
+    def kgrid_to_basis(self, kgrid, no_parity=False):
+        k = np.ascontiguousarray(kgrid, dtype=np.complex128).reshape(self.kshape)
+        out = np.empty(self.N_star)
+        self.lib.call("pscf_kgrid_to_basis", self.h, dptr(k), dptr(out), int(no_parity))
+        return out

@@ -1,0 +1,9 @@
+#!/bin/sh
+# Build the CPU emulation of libpscf_hip (test infrastructure; see hip/hip_runtime.h).
+# usage: build_emu.sh [extra g++ flags, e.g. -fsanitize=address]
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+ROOT=$(cd "$HERE/../.." && pwd)
+g++ -std=c++20 -O1 -g -fPIC -shared -pthread -Wno-unknown-pragmas "$@" -I"$HERE" \
+    -x c++ "$ROOT/pscf_amd/csrc/pscf_hip.hip" -x c++ "$ROOT/pscf_amd/csrc/am.cpp" \
+    -o "$HERE/libpscf_emu.so"

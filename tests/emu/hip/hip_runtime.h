@@ -1,0 +1,124 @@
+// CPU emulation of the small HIP subset used by pscf_amd/csrc -- TEST
+// INFRASTRUCTURE ONLY.  Compiling the library's unmodified sources with g++
+// against this header (tests/emu/build_emu.sh) runs every kernel on host
+// threads: one user-level fiber per GPU thread, __syncthreads() as a yield,
+// __shared__ as block-shared static storage (blocks run one at a time).  It
+// checks indexing, synchronisation structure and results on the CPU (also
+// under AddressSanitizer); it says nothing about GPU performance and is never
+// loaded by the product path.
+#pragma once
+#include <ucontext.h>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __shared__ static
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+
+struct double2 { double x, y; };
+inline double2 make_double2(double a, double b) { return double2{a, b}; }
+
+struct dim3 {
+  unsigned x, y, z;
+  constexpr dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+inline dim3 threadIdx;
+inline dim3 blockIdx;
+inline dim3 gridDim;
+inline dim3 blockDim;
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+enum { hipSuccess = 0, hipErrorMemoryAllocation = 2 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+
+inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated HIP error" : "no error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipMalloc(void** p, size_t n) {
+  *p = std::malloc(n ? n : 1);
+  if (*p) std::memset(*p, 0xff, n);   // poison: uninitialised reads show up as NaN
+  return *p ? hipSuccess : hipErrorMemoryAllocation;
+}
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+
+namespace emu {
+// Every GPU thread of a block is a user-level fiber on the calling OS thread.
+// __syncthreads() yields to a round-robin scheduler; one scheduler round is
+// one barrier phase.  Kernels here hit their barriers block-uniformly, which
+// is what a GPU requires as well.
+struct Fibers {
+  ucontext_t main_ctx;
+  std::vector<ucontext_t> ctx;
+  std::vector<std::vector<char>> stacks;
+  std::vector<char> finished;
+  std::function<void()> job;
+  unsigned cur = 0;
+};
+inline Fibers& fibers() { static Fibers f; return f; }
+
+inline void fiber_entry() {
+  Fibers& F = fibers();
+  F.job();
+  F.finished[F.cur] = 1;
+  swapcontext(&F.ctx[F.cur], &F.main_ctx);
+}
+
+template <class K, class... A>
+void launch(K kernel, dim3 g, dim3 b, A... args) {
+  const unsigned nt = b.x * b.y * b.z;
+  Fibers& F = fibers();
+  gridDim = g;
+  blockDim = b;
+  F.ctx.resize(nt);
+  F.stacks.resize(nt);
+  F.finished.assign(nt, 0);
+  for (auto& st : F.stacks) st.resize(256 * 1024);
+  F.job = [&] { kernel(args...); };
+  for (unsigned bz = 0; bz < g.z; ++bz)
+    for (unsigned by = 0; by < g.y; ++by)
+      for (unsigned bx = 0; bx < g.x; ++bx) {
+        blockIdx = dim3(bx, by, bz);
+        for (unsigned t = 0; t < nt; ++t) {
+          F.finished[t] = 0;
+          getcontext(&F.ctx[t]);
+          F.ctx[t].uc_stack.ss_sp = F.stacks[t].data();
+          F.ctx[t].uc_stack.ss_size = F.stacks[t].size();
+          F.ctx[t].uc_link = nullptr;
+          makecontext(&F.ctx[t], fiber_entry, 0);
+        }
+        unsigned alive = nt;
+        while (alive) {
+          alive = 0;
+          for (unsigned t = 0; t < nt; ++t) {
+            if (F.finished[t]) continue;
+            F.cur = t;
+            threadIdx = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
+            swapcontext(&F.main_ctx, &F.ctx[t]);
+            if (!F.finished[t]) ++alive;
+          }
+        }
+      }
+}
+}  // namespace emu
+
+inline void __syncthreads() {
+  emu::Fibers& F = emu::fibers();
+  swapcontext(&F.ctx[F.cur], &F.main_ctx);
+}
+
+#define hipLaunchKernelGGL(k, g, b, shm, st, ...) emu::launch(k, g, b, __VA_ARGS__)
