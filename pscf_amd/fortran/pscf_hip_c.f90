@@ -1,0 +1,231 @@
+!-----------------------------------------------------------------------
+! pscf_hip_c -- ISO_C_BINDING interfaces to libpscf_hip (include/pscf_hip.h)
+! and the state shared by the drop-in modules fft_mod, step_mod,
+! chain_mod, scf_mod and grid_basis_mod.
+!-----------------------------------------------------------------------
+module pscf_hip_c
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+   public :: hip_check
+   public :: c_pscf_fft_plan_create, c_pscf_fft_r2c, c_pscf_fft_c2r, c_pscf_fft_c2c
+   public :: c_pscf_step_create, c_pscf_step_destroy, c_pscf_step_make_propg, c_pscf_step_exp
+   public :: c_pscf_chain_discretize
+   public :: c_pscf_system_create, c_pscf_system_destroy, c_pscf_system_set_basis
+   public :: c_pscf_system_set_cell, c_pscf_system_set_chemistry, c_pscf_system_set_chi
+   public :: c_pscf_system_startup, c_pscf_density, c_pscf_scf_stress
+   public :: c_pscf_mu_phi_chain, c_pscf_mu_phi_solvent, c_pscf_free_energy
+   public :: c_pscf_free_energy_fh, c_pscf_set_omega_uniform, c_pscf_divide_energy_terms
+   public :: c_pscf_system_get_grids, c_pscf_basis_to_kgrid, c_pscf_kgrid_to_basis
+
+   interface
+      function c_pscf_last_error() bind(C, name='pscf_last_error')
+         import :: c_ptr
+         type(c_ptr) :: c_pscf_last_error
+      end function
+
+      integer(c_int) function c_pscf_fft_plan_create(ngrid, plan) bind(C, name='pscf_fft_plan_create')
+         import :: c_int, c_int64_t
+         integer(c_int), intent(IN) :: ngrid(3)
+         integer(c_int64_t), intent(OUT) :: plan
+      end function
+      integer(c_int) function c_pscf_fft_r2c(plan, in, out) bind(C, name='pscf_fft_r2c')
+         import :: c_int, c_int64_t, c_double, c_double_complex
+         integer(c_int64_t), value :: plan
+         real(c_double), intent(IN) :: in(*)
+         complex(c_double_complex), intent(OUT) :: out(*)
+      end function
+      integer(c_int) function c_pscf_fft_c2r(plan, in, out) bind(C, name='pscf_fft_c2r')
+         import :: c_int, c_int64_t, c_double, c_double_complex
+         integer(c_int64_t), value :: plan
+         complex(c_double_complex), intent(IN) :: in(*)
+         real(c_double), intent(OUT) :: out(*)
+      end function
+      integer(c_int) function c_pscf_fft_c2c(plan, direction, in, out) bind(C, name='pscf_fft_c2c')
+         import :: c_int, c_int64_t, c_double_complex
+         integer(c_int64_t), value :: plan
+         integer(c_int), value :: direction
+         complex(c_double_complex), intent(IN) :: in(*)
+         complex(c_double_complex), intent(OUT) :: out(*)
+      end function
+
+      integer(c_int) function c_pscf_step_create(ngrid, step) bind(C, name='pscf_step_create')
+         import :: c_int, c_int64_t
+         integer(c_int), intent(IN) :: ngrid(3)
+         integer(c_int64_t), intent(OUT) :: step
+      end function
+      integer(c_int) function c_pscf_step_destroy(step) bind(C, name='pscf_step_destroy')
+         import :: c_int, c_int64_t
+         integer(c_int64_t), value :: step
+      end function
+      integer(c_int) function c_pscf_step_make_propg(step, ds, b, ksq, omega) &
+            bind(C, name='pscf_step_make_propg')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: step
+         real(c_double), value :: ds, b
+         real(c_double), intent(IN) :: ksq(*), omega(*)
+      end function
+      integer(c_int) function c_pscf_step_exp(step, q_in, q_out) bind(C, name='pscf_step_exp')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: step
+         real(c_double), intent(IN) :: q_in(*)
+         real(c_double), intent(OUT) :: q_out(*)
+      end function
+
+      integer(c_int) function c_pscf_chain_discretize(nblk, blk_length, ds0, order, block_bgn, &
+            block_ds) bind(C, name='pscf_chain_discretize')
+         import :: c_int, c_double
+         integer(c_int), value :: nblk, order
+         real(c_double), intent(IN) :: blk_length(*)
+         real(c_double), value :: ds0
+         integer(c_int), intent(OUT) :: block_bgn(*)
+         real(c_double), intent(OUT) :: block_ds(*)
+      end function
+
+      integer(c_int) function c_pscf_system_create(ngrid, dim, sys) bind(C, name='pscf_system_create')
+         import :: c_int, c_int64_t
+         integer(c_int), intent(IN) :: ngrid(3)
+         integer(c_int), value :: dim
+         integer(c_int64_t), intent(OUT) :: sys
+      end function
+      integer(c_int) function c_pscf_system_destroy(sys) bind(C, name='pscf_system_destroy')
+         import :: c_int, c_int64_t
+         integer(c_int64_t), value :: sys
+      end function
+      integer(c_int) function c_pscf_system_set_basis(sys, n_wave, n_star, wave, coeff, &
+            star_of_wave, star_invert, wave_of_star) bind(C, name='pscf_system_set_basis')
+         import :: c_int, c_int64_t, c_double_complex
+         integer(c_int64_t), value :: sys
+         integer(c_int), value :: n_wave, n_star
+         integer(c_int), intent(IN) :: wave(*), star_of_wave(*), star_invert(*), wave_of_star(*)
+         complex(c_double_complex), intent(IN) :: coeff(*)
+      end function
+      integer(c_int) function c_pscf_system_set_cell(sys, G_basis) bind(C, name='pscf_system_set_cell')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: G_basis(*)
+      end function
+      integer(c_int) function c_pscf_system_set_chemistry(sys, n_monomer, kuhn, n_chain, n_block, &
+            n_blk_max, block_monomer, block_length, n_solvent, solvent_monomer, solvent_size, &
+            ensemble) bind(C, name='pscf_system_set_chemistry')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         integer(c_int), value :: n_monomer, n_chain, n_blk_max, n_solvent, ensemble
+         real(c_double), intent(IN) :: kuhn(*), block_length(*), solvent_size(*)
+         integer(c_int), intent(IN) :: n_block(*), block_monomer(*), solvent_monomer(*)
+      end function
+      integer(c_int) function c_pscf_system_set_chi(sys, chi) bind(C, name='pscf_system_set_chi')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: chi(*)
+      end function
+      integer(c_int) function c_pscf_system_startup(sys, chain_step) bind(C, name='pscf_system_startup')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), value :: chain_step
+      end function
+      integer(c_int) function c_pscf_density(sys, omega, rho, qout, q_solvent, phi_chain, &
+            mu_chain, phi_solvent, mu_solvent) bind(C, name='pscf_density')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: omega(*)
+         real(c_double), intent(OUT) :: rho(*)
+         real(c_double), intent(INOUT) :: qout(*), q_solvent(*), phi_chain(*), mu_chain(*)
+         real(c_double), intent(INOUT) :: phi_solvent(*), mu_solvent(*)
+      end function
+      integer(c_int) function c_pscf_scf_stress(sys, ncp, dGsq, phi_chain, mu_chain, stress) &
+            bind(C, name='pscf_scf_stress')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         integer(c_int), value :: ncp
+         real(c_double), intent(IN) :: dGsq(*), phi_chain(*), mu_chain(*)
+         real(c_double), intent(OUT) :: stress(*)
+      end function
+      integer(c_int) function c_pscf_mu_phi_chain(sys, mu, phi, q) bind(C, name='pscf_mu_phi_chain')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(INOUT) :: mu(*), phi(*)
+         real(c_double), intent(IN) :: q(*)
+      end function
+      integer(c_int) function c_pscf_mu_phi_solvent(sys, mu, phi, q) bind(C, name='pscf_mu_phi_solvent')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(INOUT) :: mu(*), phi(*)
+         real(c_double), intent(IN) :: q(*)
+      end function
+      integer(c_int) function c_pscf_free_energy(sys, rho, omega, phi_chain, mu_chain, &
+            phi_solvent, mu_solvent, f, p) bind(C, name='pscf_free_energy')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: rho(*), omega(*), phi_chain(*), mu_chain(*)
+         real(c_double), intent(IN) :: phi_solvent(*), mu_solvent(*)
+         real(c_double), intent(OUT) :: f, p
+      end function
+      integer(c_int) function c_pscf_free_energy_fh(sys, phi_chain, phi_solvent, f) &
+            bind(C, name='pscf_free_energy_fh')
+         import :: c_int, c_int64_t, c_double, c_ptr
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: phi_chain(*)
+         type(c_ptr), value :: phi_solvent
+         real(c_double), intent(OUT) :: f
+      end function
+      integer(c_int) function c_pscf_set_omega_uniform(sys, omega, phi_chain, phi_solvent) &
+            bind(C, name='pscf_set_omega_uniform')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(INOUT) :: omega(*)
+         real(c_double), intent(IN) :: phi_chain(*), phi_solvent(*)
+      end function
+      integer(c_int) function c_pscf_divide_energy_terms(sys, out) &
+            bind(C, name='pscf_divide_energy_terms')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(OUT) :: out(*)
+      end function
+      integer(c_int) function c_pscf_system_get_grids(sys, omega_grid, rho_grid) &
+            bind(C, name='pscf_system_get_grids')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(OUT) :: omega_grid(*), rho_grid(*)
+      end function
+      integer(c_int) function c_pscf_basis_to_kgrid(sys, basis, kgrid, no_parity) &
+            bind(C, name='pscf_basis_to_kgrid')
+         import :: c_int, c_int64_t, c_double, c_double_complex
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(IN) :: basis(*)
+         complex(c_double_complex), intent(OUT) :: kgrid(*)
+         integer(c_int), value :: no_parity
+      end function
+      integer(c_int) function c_pscf_kgrid_to_basis(sys, kgrid, basis, no_parity) &
+            bind(C, name='pscf_kgrid_to_basis')
+         import :: c_int, c_int64_t, c_double, c_double_complex
+         integer(c_int64_t), value :: sys
+         complex(c_double_complex), intent(IN) :: kgrid(*)
+         real(c_double), intent(OUT) :: basis(*)
+         integer(c_int), value :: no_parity
+      end function
+   end interface
+
+contains
+
+   ! A nonzero status stops the program with the library's message, as the
+   ! reference stops on its own errors.
+   subroutine hip_check(status, where)
+      integer(c_int), intent(IN) :: status
+      character(*), intent(IN)   :: where
+      character(kind=c_char), pointer :: msg(:)
+      type(c_ptr) :: p
+      integer :: n
+      if (status == 0) return
+      p = c_pscf_last_error()
+      call c_f_pointer(p, msg, [512])
+      n = 0
+      do while (n < 512)
+         if (msg(n+1) == c_null_char) exit
+         n = n + 1
+      end do
+      write(6,'(A,": ",512A)') trim(where), msg(1:n)
+      stop 'libpscf_hip error'
+   end subroutine hip_check
+
+end module pscf_hip_c

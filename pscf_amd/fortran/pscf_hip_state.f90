@@ -1,0 +1,88 @@
+!-----------------------------------------------------------------------
+! pscf_hip_state -- the device system shared by the drop-in scf_mod and
+! grid_basis_mod.  The reference keeps this state in module globals
+! (src/scf_mod.fp.f90:53-55, src/grid_mod.f90:43-46, src/basis_mod.f90:60-84);
+! here one libpscf_hip system mirrors them: the star tables are uploaded once
+! after make_basis, the cell whenever unit_cell_mod's G_basis changes (the
+! driver calls make_unit_cell/make_ksq itself: src/iterate_mod.fp.f90:1453-1459),
+! and the chemistry at density_startup and before every density().
+!-----------------------------------------------------------------------
+module pscf_hip_state
+   use, intrinsic :: iso_c_binding
+   use const_mod
+   use pscf_hip_c
+   implicit none
+   private
+   public :: hsys, ensure_system, sync_cell, sync_chemistry, sync_chi
+
+   integer(c_int64_t), save :: hsys = 0
+   real(long), save         :: last_G(3,3) = 0.0_long
+   logical, save            :: have_cell = .false.
+
+contains
+
+   subroutine ensure_system()
+      use grid_mod,  only : ngrid
+      use basis_mod, only : N_wave, N_star, wave, coeff, star_of_wave, star_invert, &
+                            wave_of_star
+      integer(c_int) :: ng(3)
+      integer(c_int), allocatable :: w(:,:), ws(:,:)
+      if (hsys /= 0) return
+      ng = ngrid
+      call hip_check(c_pscf_system_create(ng, int(dim, c_int), hsys), 'pscf_system_create')
+      allocate(w(dim, N_wave), ws(dim, N_star))
+      w = wave(1:dim, 1:N_wave)
+      ws = wave_of_star(1:dim, 1:N_star)
+      call hip_check(c_pscf_system_set_basis(hsys, int(N_wave, c_int), int(N_star, c_int), w, &
+                     coeff(1:N_wave), star_of_wave(1:N_wave), star_invert(1:N_star), ws), &
+                     'pscf_system_set_basis')
+      deallocate(w, ws)
+      have_cell = .false.
+      call sync_cell()
+      call sync_chemistry()
+   end subroutine ensure_system
+
+   ! upload G_basis when it differs from the last upload
+   subroutine sync_cell()
+      use unit_cell_mod, only : G_basis
+      real(c_double) :: g(dim, dim)
+      if (have_cell) then
+         if (all(G_basis(1:dim,1:dim) == last_G(1:dim,1:dim))) return
+      end if
+      g = G_basis(1:dim, 1:dim)
+      call hip_check(c_pscf_system_set_cell(hsys, g), 'pscf_system_set_cell')
+      last_G(1:dim,1:dim) = G_basis(1:dim,1:dim)
+      have_cell = .true.
+   end subroutine sync_cell
+
+   subroutine sync_chemistry()
+      use chemistry_mod, only : N_monomer, kuhn, N_chain, N_block, N_blk_max, block_monomer, &
+                                block_length, N_solvent, solvent_monomer, solvent_size, ensemble
+      integer(c_int)  :: nb(max(N_chain,1)), bm(max(N_blk_max,1), max(N_chain,1))
+      integer(c_int)  :: sm(max(N_solvent,1))
+      real(c_double)  :: bl(max(N_blk_max,1), max(N_chain,1)), ss(max(N_solvent,1))
+      nb = 1; bm = 1; bl = 0.0_long; sm = 1; ss = 0.0_long
+      if (N_chain > 0) then
+         nb(1:N_chain) = N_block(1:N_chain)
+         bm(1:N_blk_max, 1:N_chain) = block_monomer(1:N_blk_max, 1:N_chain)
+         bl(1:N_blk_max, 1:N_chain) = block_length(1:N_blk_max, 1:N_chain)
+      end if
+      if (N_solvent > 0) then
+         sm(1:N_solvent) = solvent_monomer(1:N_solvent)
+         ss(1:N_solvent) = solvent_size(1:N_solvent)
+      end if
+      call hip_check(c_pscf_system_set_chemistry(hsys, int(N_monomer, c_int), kuhn(1:N_monomer), &
+                     int(N_chain, c_int), nb, int(max(N_blk_max,1), c_int), bm, bl, &
+                     int(N_solvent, c_int), sm, ss, int(ensemble, c_int)), &
+                     'pscf_system_set_chemistry')
+      call sync_chi()
+   end subroutine sync_chemistry
+
+   subroutine sync_chi()
+      use chemistry_mod, only : N_monomer, chi
+      real(c_double) :: c(N_monomer, N_monomer)
+      c = chi(1:N_monomer, 1:N_monomer)
+      call hip_check(c_pscf_system_set_chi(hsys, c), 'pscf_system_set_chi')
+   end subroutine sync_chi
+
+end module pscf_hip_state

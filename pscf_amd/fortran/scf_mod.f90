@@ -1,0 +1,250 @@
+!-----------------------------------------------------------------------
+! scf_mod -- drop-in replacement of src/scf_mod.fp.f90 with the same public
+! procedures, arguments and module variable `plan`.  density() and
+! scf_stress() run the whole SCFT inner loop on the GPU (omega stars ->
+! grid, both contour sweeps with the contour slabs resident in HBM, Q,
+! Simpson rho, rho grid -> stars; stress from every contour point);
+! thermodynamics are libpscf_hip's host FP64 restatements.
+!
+! Side effects kept from the reference: density() leaves the omega and rho
+! grid fields in grid_mod's omega_grid/rho_grid (src/scf_mod.fp.f90:182-218)
+! when sync_grid_globals is .true. (the default), and updates phi_chain /
+! phi_solvent through mu_phi_* in the grand ensemble (:202-209).
+!-----------------------------------------------------------------------
+module scf_mod
+   use, intrinsic :: iso_c_binding
+   use const_mod
+   use chemistry_mod
+   use fft_mod
+   use grid_mod
+   use grid_basis_mod
+   use chain_mod
+   use step_mod
+   use pscf_hip_c
+   use pscf_hip_state
+   implicit none
+
+   private
+
+   public:: density_startup
+   public:: density
+   public:: scf_stress
+   public:: mu_phi_chain
+   public:: mu_phi_solvent
+   public:: free_energy
+   public:: free_energy_FH
+   public:: set_omega_uniform
+   public:: divide_energy
+   public:: plan
+   public:: sync_grid_globals
+
+   type(fft_plan)  :: plan
+   integer         :: extrap_order
+   logical, save   :: sync_grid_globals = .true.
+
+contains
+
+   !--------------------------------------------------------------------
+   ! density_startup(N_grids, extr_order, chain_step, update_chain)
+   ! src/scf_mod.fp.f90:82-117
+   !--------------------------------------------------------------------
+   subroutine density_startup(N_grids, extr_order, chain_step, update_chain)
+   integer, intent(IN)    :: N_grids(3)
+   integer, intent(IN)    :: extr_order
+   real(long), intent(IN) :: chain_step
+   logical, intent(IN)    :: update_chain
+   if (.not. update_chain) then
+      call create_fft_plan(N_grids, plan)
+      call init_step(N_grids)
+      extrap_order = extr_order
+   end if
+   call ensure_system()
+   call sync_cell()
+   call sync_chemistry()
+   call hip_check(c_pscf_system_startup(hsys, chain_step), 'density_startup')
+   end subroutine density_startup
+
+   !--------------------------------------------------------------------
+   ! density(N, omega, rho, qout, q_solvent)  src/scf_mod.fp.f90:144-229
+   !--------------------------------------------------------------------
+   subroutine density(N, omega, rho, qout, q_solvent)
+   integer,    intent(IN)            :: N
+   real(long), intent(IN)            :: omega(:,:)
+   real(long), intent(OUT)           :: rho(:,:)
+   real(long), intent(OUT), optional :: qout(N_chain)
+   real(long), intent(OUT), optional :: q_solvent(N_solvent)
+   real(c_double) :: q(max(N_chain,1)), qs(max(N_solvent,1))
+   real(c_double) :: phc(max(N_chain,1)), muc(max(N_chain,1))
+   real(c_double) :: phs(max(N_solvent,1)), mus(max(N_solvent,1))
+   call sync_cell()
+   call sync_chi()
+   phc = 0.0_long; muc = 0.0_long; phs = 0.0_long; mus = 0.0_long
+   if (N_chain > 0) then
+      phc(1:N_chain) = phi_chain(1:N_chain)
+      muc(1:N_chain) = mu_chain(1:N_chain)
+   end if
+   if (N_solvent > 0) then
+      phs(1:N_solvent) = phi_solvent(1:N_solvent)
+      mus(1:N_solvent) = mu_solvent(1:N_solvent)
+   end if
+   call hip_check(c_pscf_density(hsys, omega, rho, q, qs, phc, muc, phs, mus), 'density')
+   if (present(qout)) qout(1:N_chain) = q(1:N_chain)
+   if (present(q_solvent)) q_solvent(1:N_solvent) = qs(1:N_solvent)
+   if (ensemble == 1) then
+      if (present(qout)) phi_chain(1:N_chain) = phc(1:N_chain)
+      if (present(q_solvent)) phi_solvent(1:N_solvent) = phs(1:N_solvent)
+   end if
+   if (sync_grid_globals) then
+      call hip_check(c_pscf_system_get_grids(hsys, omega_grid, rho_grid), 'density')
+   end if
+   end subroutine density
+
+   !--------------------------------------------------------------------
+   ! scf_stress(N, size_dGsq, dGsq)            src/scf_mod.fp.f90:451-550
+   !--------------------------------------------------------------------
+   function scf_stress(N, size_dGsq, dGsq)
+   integer,    intent(IN) :: N
+   integer,    intent(IN) :: size_dGsq
+   real(long), intent(IN) :: dGsq(:,:)
+   real(long)             :: scf_stress(size_dGsq)
+   real(c_double) :: phc(max(N_chain,1)), muc(max(N_chain,1))
+   phc = 0.0_long; muc = 0.0_long
+   if (N_chain > 0) then
+      phc(1:N_chain) = phi_chain(1:N_chain)
+      muc(1:N_chain) = mu_chain(1:N_chain)
+   end if
+   call hip_check(c_pscf_scf_stress(hsys, int(size_dGsq, c_int), dGsq, phc, muc, scf_stress), &
+                  'scf_stress')
+   end function scf_stress
+
+   !--------------------------------------------------------------------
+   ! divide_energy (DEVEL variant)              src/scf_mod.fp.f90:588-709
+   !--------------------------------------------------------------------
+   subroutine divide_energy(rho, omega, phi_chain, phi_solvent, Q, f_tot, f_comp, ovlap)
+   real(long), intent(IN)  :: rho(:,:)
+   real(long), intent(IN)  :: omega(:,:)
+   real(long), intent(IN)  :: phi_chain(:)
+   real(long), intent(IN)  :: phi_solvent(:)
+   real(long), intent(IN)  :: Q(:)
+   real(long), intent(IN)  :: f_tot
+   real(long), intent(OUT) :: f_comp(:)
+   real(long), intent(OUT) :: ovlap(:,:)
+   real(long) :: enthalpy, fhead, ftail, fjct
+   real(c_double) :: terms(2*max(N_chain,1))
+   integer :: alpha, beta, i
+   enthalpy = 0.0_long
+   ovlap = 0.0_long
+   do alpha = 1, N_monomer
+      do beta = alpha+1, N_monomer
+         ovlap(alpha,beta) = dot_product(rho(alpha,:), rho(beta,:))
+         ovlap(beta,alpha) = ovlap(alpha,beta)
+         enthalpy = enthalpy + chi(alpha,beta) * ovlap(alpha,beta)
+      end do
+   end do
+   call hip_check(c_pscf_divide_energy_terms(hsys, terms), 'divide_energy')
+   fhead = 0.0_long
+   ftail = 0.0_long
+   do i = 1, N_chain
+      fhead = fhead - terms(2*i-1) * phi_chain(i) / chain_length(i)
+      ftail = ftail - terms(2*i)   * phi_chain(i) / chain_length(i)
+   end do
+   do i = 1, N_chain
+      beta = block_monomer(1,i)
+      fhead = fhead - dot_product(omega(beta,:), rho(beta,:)) * phi_chain(i)
+      beta = block_monomer(N_block(i),i)
+      ftail = ftail - dot_product(omega(beta,:), rho(beta,:)) * phi_chain(i)
+   end do
+   fjct = 0.0_long
+   do i = 1, N_chain
+      fjct = fjct + phi_chain(i) / chain_length(i)
+   end do
+   fjct = f_tot + fjct - enthalpy - fhead - ftail
+   f_comp(1) = enthalpy
+   f_comp(2) = fhead
+   f_comp(3) = ftail
+   f_comp(4) = fjct
+   end subroutine divide_energy
+
+   ! set_omega_uniform(omega)                   src/scf_mod.fp.f90:724-750
+   subroutine set_omega_uniform(omega)
+   real(long), intent(INOUT) :: omega(:,:)
+   real(c_double) :: phc(max(N_chain,1)), phs(max(N_solvent,1))
+   phc = 0.0_long; phs = 0.0_long
+   if (N_chain > 0) phc(1:N_chain) = phi_chain(1:N_chain)
+   if (N_solvent > 0) phs(1:N_solvent) = phi_solvent(1:N_solvent)
+   call ensure_system()
+   call sync_chi()
+   call hip_check(c_pscf_set_omega_uniform(hsys, omega, phc, phs), 'set_omega_uniform')
+   end subroutine set_omega_uniform
+
+   ! mu_phi_chain(mu, phi, q)                   src/scf_mod.fp.f90:768-785
+   subroutine mu_phi_chain(mu, phi, q)
+   real(long), intent(INOUT) :: mu(N_chain)
+   real(long), intent(INOUT) :: phi(N_chain)
+   real(long), intent(IN)    :: q(N_chain)
+   if (N_chain == 0) return
+   call ensure_system()
+   call hip_check(c_pscf_mu_phi_chain(hsys, mu, phi, q), 'mu_phi_chain')
+   end subroutine mu_phi_chain
+
+   ! mu_phi_solvent(mu, phi, q)                 src/scf_mod.fp.f90:803-820
+   subroutine mu_phi_solvent(mu, phi, q)
+   real(long), intent(INOUT) :: mu(N_solvent)
+   real(long), intent(INOUT) :: phi(N_solvent)
+   real(long), intent(IN)    :: q(N_solvent)
+   if (N_solvent == 0) return
+   call ensure_system()
+   call hip_check(c_pscf_mu_phi_solvent(hsys, mu, phi, q), 'mu_phi_solvent')
+   end subroutine mu_phi_solvent
+
+   ! free_energy(...)                           src/scf_mod.fp.f90:834-886
+   subroutine free_energy(N, rho, omega, phi_chain, mu_chain, &
+                          phi_solvent, mu_solvent, f_Helmholtz, pressure)
+   integer, intent(IN)    :: N
+   real(long), intent(IN) :: rho(:,:)
+   real(long), intent(IN) :: omega(:,:)
+   real(long), intent(IN) :: phi_chain(:)
+   real(long), intent(IN) :: mu_chain(:)
+   real(long), intent(IN) :: phi_solvent(:)
+   real(long), intent(IN) :: mu_solvent(:)
+   real(long), intent(OUT):: f_Helmholtz
+   real(long), intent(OUT), optional :: pressure
+   real(c_double) :: phc(max(N_chain,1)), muc(max(N_chain,1))
+   real(c_double) :: phs(max(N_solvent,1)), mus(max(N_solvent,1)), p
+   phc = 0.0_long; muc = 0.0_long; phs = 0.0_long; mus = 0.0_long
+   if (N_chain > 0) then
+      phc(1:N_chain) = phi_chain(1:N_chain)
+      muc(1:N_chain) = mu_chain(1:N_chain)
+   end if
+   if (N_solvent > 0) then
+      phs(1:N_solvent) = phi_solvent(1:N_solvent)
+      mus(1:N_solvent) = mu_solvent(1:N_solvent)
+   end if
+   call ensure_system()
+   call sync_chi()
+   call hip_check(c_pscf_free_energy(hsys, rho, omega, phc, muc, phs, mus, f_Helmholtz, p), &
+                  'free_energy')
+   if (present(pressure)) pressure = p
+   end subroutine free_energy
+
+   ! free_energy_FH(phi_chain, phi_solvent)     src/scf_mod.fp.f90:903-941
+   real(long) function free_energy_FH(phi_chain, phi_solvent)
+   real(long), intent(IN)           :: phi_chain(N_chain)
+   real(long), intent(IN), optional :: phi_solvent(N_solvent)
+   real(c_double)         :: phc(max(N_chain,1)), f
+   real(c_double), target :: phs(max(N_solvent,1))
+   phc = 0.0_long
+   if (N_chain > 0) phc(1:N_chain) = phi_chain(1:N_chain)
+   call ensure_system()
+   call sync_chi()
+   if (present(phi_solvent)) then
+      phs = 0.0_long
+      if (N_solvent > 0) phs(1:N_solvent) = phi_solvent(1:N_solvent)
+      call hip_check(c_pscf_free_energy_fh(hsys, phc, c_loc(phs), f), 'free_energy_FH')
+   else
+      call hip_check(c_pscf_free_energy_fh(hsys, phc, c_null_ptr, f), 'free_energy_FH')
+   end if
+   free_energy_FH = f
+   end function free_energy_FH
+
+end module scf_mod

@@ -1,0 +1,114 @@
+"""The drop-in boundary end to end: the reference PSCF program's own Fortran
+driver (main program, iterate_AM, field I/O -- compiled unchanged from the
+reference sources) linked against the replacement modules of
+pscf_amd/fortran (fft_mod, step_mod, chain_mod, scf_mod, grid_basis_mod),
+binary pscf_amd/fortran/_build/pscf_gpu, against the reference program
+itself (oracle/_ref/pscf) on identical input files.
+
+CPU: the GPU library is replaced by its CPU-emulated build (LD_PRELOAD), and
+the first Anderson iterations' residual / free-energy log must agree with the
+reference's.  GPU: both run to convergence and must reproduce the known
+answers that the reference ships (tools/python/tests/outfile/in:39-79,
+lamellar f=0.5, chi N=15).
+"""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import EMU_LIB, GOLDEN, ROOT, get_lib, load_golden
+
+sys.path.insert(0, GOLDEN)
+from make_golden import cases  # noqa: E402
+
+from pscf_amd import fieldio, scf  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "pscf")
+GPU_BIN = os.path.join(ROOT, "pscf_amd", "fortran", "_build", "pscf_gpu")
+
+# known answers of the reference (tools/python/tests/outfile/in:70-77)
+KNOWN = {"f_Helmholtz": 2.4293253659, "pressure": 3.0121269977, "mu_chain": 5.4414523636}
+
+
+def _need(path):
+    if not os.path.exists(path):
+        pytest.skip("%s not built (make -C oracle; make -C pscf_amd/fortran)" % path)
+
+
+def _write_case(tmp_path, max_itr):
+    g = load_golden("lam32")
+    sysm = cases()["lam32"][0]
+    B = scf.Basis.from_tables(g)
+    om = np.zeros((2, B.N_star))
+    om[0, 1], om[1, 1] = 3.0, -3.0
+    fieldio.write_field(str(tmp_path / "in.omega"), om, sysm, B, g["star_count"])
+    text = sysm.to_param() + (
+        "ITERATE\ninput_filename\n'in.omega'\noutput_prefix\n'out/'\nmax_itr\n%d\n"
+        "error_max\n1.0E-10\ndomain\nF\nitr_algo\n'AM'\nN_hist\n10\n\nFINISH\n" % max_itr)
+    (tmp_path / "param").write_text(text)
+    (tmp_path / "out").mkdir(exist_ok=True)
+
+
+def _run(binary, cwd, env=None):
+    with open(os.path.join(cwd, "param")) as f:
+        r = subprocess.run([binary], stdin=f, cwd=cwd, capture_output=True, text=True, env=env,
+                           timeout=900)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def _log(text):
+    """(iteration, residual, f_Helmholtz) per iteration of iterate_AM's log."""
+    its = re.findall(r"Iteration\s+(\d+)\s*\nmax \(SCF \+ 100xStress\) residuals\s*=\s*(\S+)"
+                     r"[\s\S]*?f_Helmholtz =\s*(\S+)", text)
+    return [(int(i), float(e), float(f)) for i, e, f in its]
+
+
+def _summary(path):
+    txt = open(path).read()
+    out = {}
+    for key in ("f_Helmholtz", "pressure", "mu_chain", "iterations"):
+        m = re.search(r"^%s\s*\n\s*(\S+)" % key, txt, re.M)
+        out[key] = float(m.group(1))
+    return out
+
+
+def test_dropin_trajectory_matches_reference_cpu(tmp_path):
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("emu")            # builds the emulated library if needed
+    _write_case(tmp_path, max_itr=4)
+    ref = _log(_run(REF, str(tmp_path)))
+    env = dict(os.environ, LD_PRELOAD=EMU_LIB)
+    mine = _log(_run(GPU_BIN, str(tmp_path), env=env))
+    assert len(ref) >= 4 and len(mine) == len(ref)
+    for (i, e, f), (j, e2, f2) in zip(ref, mine):
+        assert i == j
+        assert abs(e2 / e - 1) < 1e-8, (i, e, e2)
+        assert abs(f2 / f - 1) < 1e-9, (i, f, f2)
+
+
+@pytest.mark.gpu
+def test_dropin_converges_to_known_answer_gpu(tmp_path):
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("gpu")
+    _write_case(tmp_path, max_itr=300)
+    ref_dir = tmp_path / "ref"
+    gpu_dir = tmp_path / "gpu"
+    for d in (ref_dir, gpu_dir):
+        d.mkdir()
+        for f in ("param", "in.omega"):
+            (d / f).write_text((tmp_path / f).read_text())
+        (d / "out").mkdir()
+    _run(REF, str(ref_dir))
+    _run(GPU_BIN, str(gpu_dir))
+    a = _summary(str(ref_dir / "out" / "out"))
+    b = _summary(str(gpu_dir / "out" / "out"))
+    for k, v in KNOWN.items():
+        assert abs(a[k] / v - 1) < 2e-10, ("reference", k, a[k], v)
+        assert abs(b[k] / v - 1) < 2e-10, ("gpu drop-in", k, b[k], v)
+        assert abs(b[k] / a[k] - 1) < 1e-9, (k, a[k], b[k])
