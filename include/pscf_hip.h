@@ -149,6 +149,19 @@ int pscf_divide_energy_terms(int64_t sys, double* out);
  * (N1,N2,N3,N_monomer) (src/grid_mod.f90:44-45); either may be NULL. */
 int pscf_system_get_grids(int64_t sys, double* omega_grid, double* rho_grid);
 
+/* Timing of the contour sweeps with HIP events on the system's stream.
+ * Returns, accumulated since the previous call and then reset, ms_sum[5] and
+ * count[5]: per step kernel (0 A: x,y passes that finish/start a step; 1 Z0:
+ * z passes of the full and first half step; 2 C: x,y passes between them;
+ * 3 Z1: z pass of the second half step) the milliseconds and launches, and
+ * in slot 4 the milliseconds of whole sweeps and the contour steps they
+ * advanced.  mode for the following density() calls: 0 off; 1 events around
+ * the captured sweep graph (slot 4 only); 2 eager launches with an event
+ * around every kernel (all slots). */
+int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count);
+/* Capture the sweeps of density() in a HIP graph (default on). */
+int pscf_system_use_graph(int64_t sys, int enable);
+
 /* Copy contour point s (0-based) of chain c's qf (dir=+1) or qr (dir=-1). */
 int pscf_system_get_q(int64_t sys, int chain, int dir, int s, double* q);
 

@@ -54,6 +54,8 @@ SIGNATURES = {
     "pscf_divide_energy_terms": [ctypes.c_int64, _c_dbl_p],
     "pscf_system_get_grids": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
     "pscf_system_get_q": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p],
+    "pscf_profile_sweeps": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_i64_p],
+    "pscf_system_use_graph": [ctypes.c_int64, ctypes.c_int],
     "pscf_basis_to_kgrid": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_kgrid_to_basis": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_unit_cell": [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,

@@ -208,10 +208,10 @@ __global__ void __launch_bounds__(NTA) k_partial_sum_exp(const double* __restric
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(NTA) k_solvent_rho(const double* __restrict__ w, double size,
                                                      const double* __restrict__ qs, int ensemble,
-                                                     double phi, double mu, long n,
+                                                     const double* __restrict__ phimu, long n,
                                                      double* __restrict__ rho) {
   const double Q = qs[0];
-  const double ph = ensemble == 1 ? Q * exp(mu) : phi;
+  const double ph = ensemble == 1 ? Q * exp(phimu[1]) : phimu[0];
   for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA)
     rho[i] = rho[i] + ph * exp((-size) * w[i]) / Q;
 }
@@ -228,16 +228,16 @@ struct SimpsonArgs {
   double ds[16];
   int monomer[16];       // 0-based
   double chain_length;
-  double phi, mu;
   int ensemble;
 };
 
 __global__ void __launch_bounds__(NTA) k_simpson(const double* __restrict__ qf,
                                                  const double* __restrict__ qr, long m,
                                                  SimpsonArgs a, const double* __restrict__ Qp,
+                                                 const double* __restrict__ phimu,
                                                  double* __restrict__ rho_grid) {
   const double Q = Qp[0];
-  const double phi = a.ensemble == 1 ? Q * exp(a.mu) : a.phi;
+  const double phi = a.ensemble == 1 ? Q * exp(phimu[1]) : phimu[0];
   for (long r = blockIdx.x * (long)NTA + threadIdx.x; r < m; r += (long)gridDim.x * NTA) {
     for (int b = 0; b < a.nblk; ++b) {
       const int ibgn = a.bnd[b], iend = a.bnd[b + 1];

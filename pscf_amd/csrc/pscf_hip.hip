@@ -294,13 +294,31 @@ struct System {
   std::vector<std::unique_ptr<ChainDev>> chains;
   bool have_chem = false, started = false;
   // work
-  DevBuf<double> omega_in, rho_out, omega_grid, rho_grid, part, qdev, basis_tmp;
+  DevBuf<double> omega_in, rho_out, omega_grid, rho_grid, part, qdev, basis_tmp, chem_dev;
   DevBuf<double2> kwork;
   DevBuf<double2> abc;     // 3 * mk per task
   DevBuf<double> q1;       // m per task
   DevBuf<StepTask> tasks;
   std::vector<double> h_Q, h_Qs;
   bool have_density = false;
+  // contour sweeps of one density() captured once as a HIP graph
+  int ntask = 0, max_ns = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool use_graph = true;
+  // optional timing (HIP events on the system stream): mode 1 brackets the
+  // sweep graph (time per contour step); mode 2 launches the sweeps eagerly
+  // with an event around every kernel (per-kernel durations)
+  int profile = 0;
+  std::vector<hipEvent_t> ev;
+  double prof_ms[5] = {0, 0, 0, 0, 0};  // step A, Z0, C, Z1, whole sweep
+  long prof_n[5] = {0, 0, 0, 0, 0};     // launches (kinds 0-3), contour steps (4)
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
   // stress work
   DevBuf<double> sbuf;     // real chunk
   DevBuf<double2> skbuf;   // spectra chunk
@@ -346,16 +364,57 @@ void dev_sum(const double* x, long n, double scale, double* part, double* out_de
 
 // --------------------------------------------------- contour sweeps
 // Advance the given sweep tasks through all their steps (4 launches per step).
-void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, hipStream_t s) {
+// With `ev` (4*max_ns + 2 events) an event is recorded before every launch and
+// after the last, for per-kernel timing.
+void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, hipStream_t s,
+                hipEvent_t* ev = nullptr) {
+  int e = 0;
+  auto mark = [&] {
+    if (ev) HIPCHK(hipEventRecord(ev[e++], s));
+  };
   for (int j = 0; j <= max_ns; ++j) {
+    mark();
     hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1, g.tw2);
     if (j == max_ns) break;
+    mark();
     hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n2, 0, g.tw3);
+    mark();
     hipLaunchKernelGGL(g.pl->c, dim3(g.n3, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1,
                        g.tw2);
+    mark();
     hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 1), dim3(NT), 0, s, d_tasks, j, g.n2, 1, g.tw3);
   }
+  mark();
   HIPCHK(hipGetLastError());
+}
+
+// accumulate the intervals of the last profiled sweep
+void collect_profile(System& S, int max_ns) {
+  if (S.profile == 1) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+    S.prof_ms[4] += ms;
+    S.prof_n[4] += max_ns;
+    return;
+  }
+  const int nev = 4 * max_ns + 2;
+  for (int i = 0; i + 1 < nev; ++i) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, S.ev[i], S.ev[i + 1]));
+    const int kind = i % 4;      // A, Z0, C, Z1 (the final interval is an A launch)
+    S.prof_ms[kind] += ms;
+    S.prof_n[kind] += 1;
+    S.prof_ms[4] += ms;
+  }
+  S.prof_n[4] += max_ns;
+}
+
+void ensure_events(System& S, size_t n) {
+  while (S.ev.size() < n) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    S.ev.push_back(e);
+  }
 }
 
 void build_basis_maps(System& S) {
@@ -489,31 +548,13 @@ void startup(System& S, double chain_step) {
   S.rho_grid.alloc((size_t)S.nmon * g.m);
   S.part.alloc(1024);
   S.qdev.alloc(S.nchain + S.nsolv + 1);
-  S.started = true;
-  S.have_density = false;
-}
-
-void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
-             double* mu_c, double* phi_s, double* mu_s) {
-  REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
-  const Grid& g = S.g;
-  hipStream_t s = S.st.s;
-  const size_t nst = (size_t)S.nmon * S.nstar;
-  S.omega_in.upload(omega, nst, s);
-  // omega stars -> grid, no 1/M (src/scf_mod.fp.f90:182-185)
-  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p);
-  // sweeps of all chains, forward and backward together
-  std::vector<StepTask> ht(2 * S.nchain);
+  S.chem_dev.alloc(2 * (S.nchain + S.nsolv) + 2);
+  // sweep tasks: forward (qf) and backward (qr) sweep of every chain
+  std::vector<StepTask> ht(ntask);
   int max_ns = 0;
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     max_ns = std::max(max_ns, ch.ns);
-    for (int b = 0; b < ch.nblk; ++b) {
-      double* f = ch.fac.p + (size_t)b * (2 * g.m + 2 * g.mk);
-      hipLaunchKernelGGL(k_make_propg, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.ds[b],
-                         S.kuhn[ch.mon[b]], S.ksq.p, g.mk, S.omega_grid.p + (size_t)ch.mon[b] * g.m,
-                         g.m, f, f + g.m, f + 2 * g.m, f + 2 * g.m + g.mk);
-    }
     for (int d = 0; d < 2; ++d) {
       StepTask& t = ht[2 * c + d];
       std::memset(&t, 0, sizeof t);
@@ -537,14 +578,80 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
       t.q1 = S.q1.p + (size_t)ti * g.m;
     }
   }
-  HIPCHK(hipMemcpyAsync(S.tasks.p, ht.data(), sizeof(StepTask) * ht.size(), hipMemcpyHostToDevice, s));
-  run_sweeps(g, S.tasks.p, (int)ht.size(), max_ns, s);
+  if (ntask > 0)
+    HIPCHK(hipMemcpyAsync(S.tasks.p, ht.data(), sizeof(StepTask) * ntask, hipMemcpyHostToDevice,
+                          S.st.s));
+  S.st.sync();
+  S.ntask = ntask;
+  S.max_ns = max_ns;
+  S.drop_graph();
+  S.started = true;
+  S.have_density = false;
+}
+
+void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
+             double* mu_c, double* phi_s, double* mu_s) {
+  REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  const size_t nst = (size_t)S.nmon * S.nstar;
+  S.omega_in.upload(omega, nst, s);
+  // omega stars -> grid, no 1/M (src/scf_mod.fp.f90:182-185)
+  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p);
+  // propagator factors per block (make_propg, src/step_mod.f90:109-129)
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    for (int b = 0; b < ch.nblk; ++b) {
+      double* f = ch.fac.p + (size_t)b * (2 * g.m + 2 * g.mk);
+      hipLaunchKernelGGL(k_make_propg, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.ds[b],
+                         S.kuhn[ch.mon[b]], S.ksq.p, g.mk, S.omega_grid.p + (size_t)ch.mon[b] * g.m,
+                         g.m, f, f + g.m, f + 2 * g.m, f + 2 * g.m + g.mk);
+    }
+  }
+  // chemistry scalars read on the device (phi, mu per chain and solvent)
+  {
+    std::vector<double> cd(2 * (S.nchain + S.nsolv) + 2, 0.0);
+    for (int c = 0; c < S.nchain; ++c) {
+      cd[2 * c] = phi_c ? phi_c[c] : 0.0;
+      cd[2 * c + 1] = mu_c ? mu_c[c] : 0.0;
+    }
+    for (int i = 0; i < S.nsolv; ++i) {
+      cd[2 * (S.nchain + i)] = phi_s ? phi_s[i] : 0.0;
+      cd[2 * (S.nchain + i) + 1] = mu_s ? mu_s[i] : 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(S.chem_dev.p, cd.data(), sizeof(double) * cd.size(),
+                          hipMemcpyHostToDevice, s));
+  }
+  // both sweeps of every chain: 4 launches per contour step
+  const int max_ns = S.max_ns;
+  if (S.profile == 2) {
+    ensure_events(S, (size_t)4 * max_ns + 2);
+    run_sweeps(g, S.tasks.p, S.ntask, max_ns, s, S.ev.data());
+  } else {
+    if (S.profile == 1) {
+      ensure_events(S, 2);
+      HIPCHK(hipEventRecord(S.ev[0], s));
+    }
+    if (S.use_graph) {
+      if (!S.graph_exec) {
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
+        HIPCHK(hipStreamEndCapture(s, &S.graph));
+        HIPCHK(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(S.graph_exec, s));
+    } else {
+      run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
+    }
+    if (S.profile == 1) HIPCHK(hipEventRecord(S.ev[1], s));
+  }
   // Q = mean of qf at the chain end (src/scf_mod.fp.f90:359-360)
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     dev_sum(ch.qf.p + (size_t)ch.ns * g.m, g.m, 1.0 / (double)g.m, S.part.p, S.qdev.p + c, s);
   }
-  HIPCHK(hipMemsetAsync(S.rho_grid.p, 0, sizeof(double) * S.nmon * g.m, s));
+  hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)S.nmon * g.m)), dim3(NTA), 0, s, S.rho_grid.p,
+                     (long)S.nmon * g.m, 0.0);
   // solvents (src/scf_mod.fp.f90:193-200, 251-295)
   for (int i = 0; i < S.nsolv; ++i) {
     const double* w = S.omega_grid.p + (size_t)S.solv_mon[i] * g.m;
@@ -553,8 +660,8 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, s, S.part.p, nb, 1.0 / (double)g.m,
                        S.qdev.p + S.nchain + i);
     hipLaunchKernelGGL(k_solvent_rho, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, w, S.solv_size[i],
-                       S.qdev.p + S.nchain + i, S.ensemble, phi_s ? phi_s[i] : 0.0,
-                       mu_s ? mu_s[i] : 0.0, g.m, S.rho_grid.p + (size_t)S.solv_mon[i] * g.m);
+                       S.qdev.p + S.nchain + i, S.ensemble, S.chem_dev.p + 2 * (S.nchain + i), g.m,
+                       S.rho_grid.p + (size_t)S.solv_mon[i] * g.m);
   }
   // Simpson rho per chain (src/scf_mod.fp.f90:362-424, 212-218)
   for (int c = 0; c < S.nchain; ++c) {
@@ -568,11 +675,9 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
       a.monomer[b] = ch.mon[b];
     }
     a.chain_length = ch.length;
-    a.phi = phi_c ? phi_c[c] : 0.0;
-    a.mu = mu_c ? mu_c[c] : 0.0;
     a.ensemble = S.ensemble;
     hipLaunchKernelGGL(k_simpson, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a,
-                       S.qdev.p + c, S.rho_grid.p);
+                       S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
   }
   // rho grid -> stars, / M (src/scf_mod.fp.f90:221-225)
   S.rho_out.alloc(nst);
@@ -583,6 +688,7 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
   S.st.sync();
+  if (S.profile) collect_profile(S, max_ns);
   for (int c = 0; c < S.nchain; ++c)
     if (qout) qout[c] = S.h_Q[c];
   for (int i = 0; i < S.nsolv; ++i)
@@ -847,7 +953,34 @@ int pscf_system_create(const int ngrid[3], int dim, int64_t* sys) {
 }
 
 int pscf_system_destroy(int64_t sys) {
-  return guarded([&] { delete from_handle<System>(sys, "system"); });
+  return guarded([&] {
+    System* S = from_handle<System>(sys, "system");
+    for (auto e : S->ev) (void)hipEventDestroy(e);
+    S->drop_graph();
+    delete S;
+  });
+}
+
+int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(mode >= 0 && mode <= 2, "profile mode must be 0, 1 or 2");
+    for (int k = 0; k < 5; ++k) {
+      if (ms_sum) ms_sum[k] = S.prof_ms[k];
+      if (count) count[k] = S.prof_n[k];
+      S.prof_ms[k] = 0.0;
+      S.prof_n[k] = 0;
+    }
+    S.profile = mode;
+  });
+}
+
+int pscf_system_use_graph(int64_t sys, int enable) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    S.use_graph = enable != 0;
+    if (!S.use_graph) S.drop_graph();
+  });
 }
 
 int pscf_system_set_basis(int64_t sys, int n_wave, int n_star, const int* wave, const double* coeff,

@@ -247,6 +247,20 @@ class SCFSystem:
                       dptr(self.phi_solvent))
         return om.T.copy()
 
+    def profile_sweeps(self, mode):
+        """HIP-event timing of the contour sweeps.  Returns (ms[5], count[5])
+        accumulated since the previous call: step kernels A, Z0, C, Z1 (mode
+        2) and whole sweeps / contour steps in slot 4 (modes 1 and 2); sets
+        the mode for later density() calls (0 off, 1 graph-level, 2 per kernel)."""
+        ms = np.zeros(5)
+        n = np.zeros(5, dtype=np.int64)
+        self.lib.call("pscf_profile_sweeps", self.h, int(mode), dptr(ms),
+                      n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        return ms, n
+
+    def use_graph(self, enable):
+        self.lib.call("pscf_system_use_graph", self.h, int(bool(enable)))
+
     # ------------------------------------------------------- inspection
     def grids(self):
         nm = self.system.N_monomer

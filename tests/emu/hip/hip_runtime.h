@@ -55,6 +55,11 @@ inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)1; return hipSuc
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+typedef int* hipEvent_t;
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new int(0); return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
 
 namespace emu {
 // Every GPU thread of a block is a user-level fiber on the calling OS thread.
@@ -78,8 +83,26 @@ inline void fiber_entry() {
   swapcontext(&F.ctx[F.cur], &F.main_ctx);
 }
 
+// stream capture: launches are recorded and replayed by hipGraphLaunch
+inline std::vector<std::function<void()>>*& capture() {
+  static std::vector<std::function<void()>>* c = nullptr;
+  return c;
+}
+
+template <class K, class... A>
+void run_grid(K kernel, dim3 g, dim3 b, A... args);
+
 template <class K, class... A>
 void launch(K kernel, dim3 g, dim3 b, A... args) {
+  if (capture()) {
+    capture()->push_back([=] { run_grid(kernel, g, b, args...); });
+    return;
+  }
+  run_grid(kernel, g, b, args...);
+}
+
+template <class K, class... A>
+void run_grid(K kernel, dim3 g, dim3 b, A... args) {
   const unsigned nt = b.x * b.y * b.z;
   Fibers& F = fibers();
   gridDim = g;
@@ -115,6 +138,31 @@ void launch(K kernel, dim3 g, dim3 b, A... args) {
       }
 }
 }  // namespace emu
+
+struct EmuGraph { std::vector<std::function<void()>> ops; };
+typedef EmuGraph* hipGraph_t;
+typedef EmuGraph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  emu::capture() = new std::vector<std::function<void()>>();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  *g = new EmuGraph{std::move(*emu::capture())};
+  delete emu::capture();
+  emu::capture() = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, void*, size_t) {
+  *e = new EmuGraph{g->ops};
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : e->ops) f();
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
 
 inline void __syncthreads() {
   emu::Fibers& F = emu::fibers();
