@@ -201,6 +201,31 @@ int pscf_am_iterate(int64_t am, double* omega, const double* rho, const double* 
                     const double* stress, double* cell_param, double error_max, int max_itr,
                     double* error, int* status);
 
+/* ------------------------------------------------------- batched solves
+ * Many independent SCFT systems iterated to convergence together, each with
+ * its own Anderson history and unit cell (iterate_AM's control flow per
+ * instance, src/iterate_mod.fp.f90:1042-1490).  Systems with the same grid
+ * share every contour-step launch; converged instances drop out.  No
+ * reference counterpart (the reference is single-instance: module globals,
+ * src/scf_mod.fp.f90:53-55); the batch only reuses the kernels above. */
+int pscf_batch_create(int n_hist, int domain, double error_max, int max_itr, int64_t* batch);
+int pscf_batch_destroy(int64_t batch);
+/* Add a started system (set_basis, set_cell, set_chemistry, set_chi,
+ * startup) with its crystal system, cell_param(n_cell_param), initial
+ * omega(N_monomer,N_star), phi_chain(N_chain), mu_chain (may be NULL) and
+ * phi_solvent(N_solvent). index: its position in the batch. */
+int pscf_batch_add(int64_t batch, int64_t sys, const char* crystal_system, int n_cell_param,
+                   const double* cell_param, const double* omega, const double* phi_chain,
+                   const double* mu_chain, const double* phi_solvent, int* index);
+/* Iterate every instance until it converges, exceeds max_itr or diverges.
+ * iterations: batched iterations run (the largest per-instance count). */
+int pscf_batch_solve(int64_t batch, int* iterations);
+/* Results of one instance: omega, rho (N_monomer,N_star), cell_param,
+ * stress, thermo(4) = (f_Helmholtz, pressure, final error, mu_chain(1)),
+ * iterations and status (1 converged, 2 max_itr, 3 diverged). */
+int pscf_batch_result(int64_t batch, int index, double* omega, double* rho, double* cell_param,
+                      double* stress, double* thermo, int* itr, int* status);
+
 #ifdef __cplusplus
 }
 #endif

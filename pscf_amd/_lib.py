@@ -58,6 +58,13 @@ SIGNATURES = {
     "pscf_system_use_graph": [ctypes.c_int64, ctypes.c_int],
     "pscf_basis_to_kgrid": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_kgrid_to_basis": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
+    "pscf_batch_create": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_i64_p],
+    "pscf_batch_destroy": [ctypes.c_int64],
+    "pscf_batch_add": [ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int, _c_dbl_p,
+                       _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p],
+    "pscf_batch_solve": [ctypes.c_int64, _c_int_p],
+    "pscf_batch_result": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                          _c_dbl_p, _c_int_p, _c_int_p],
     "pscf_unit_cell": [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                        _c_dbl_p],
     "pscf_make_dgsq": [ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p],

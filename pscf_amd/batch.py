@@ -1,0 +1,90 @@
+"""Batched SCFT solves: many independent instances (e.g. sweep points differing
+in chi N, f, cell or phase) iterated to convergence together on one GPU.
+
+Each instance keeps its own Anderson history and unit cell (the reference's
+iterate_AM control flow per instance, src/iterate_mod.fp.f90:1042-1490);
+instances with the same grid share every contour-step launch (their sweep
+tasks ride side by side in one HIP-graph-captured launch set), and converged
+instances drop out of the work.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import dptr
+from . import scf
+
+
+@dataclass
+class BatchResult:
+    omega: np.ndarray
+    rho: np.ndarray
+    cell_param: np.ndarray
+    stress: np.ndarray
+    f_helmholtz: float
+    pressure: float
+    error: float
+    mu_chain: float
+    itr: int
+    converge: bool
+    status: int
+
+
+class BatchSolver:
+    def __init__(self, n_hist=20, domain=True, error_max=1e-8, max_itr=1000, lib=None):
+        from ._lib import lib as _default_lib
+        self.lib = lib or _default_lib()
+        h = ctypes.c_int64(0)
+        self.lib.call("pscf_batch_create", int(n_hist), int(bool(domain)), float(error_max),
+                      int(max_itr), ctypes.byref(h))
+        self.h = h.value
+        self.systems = []      # keep SCFSystem objects alive
+
+    def __del__(self):
+        if getattr(self, "h", 0):
+            self.lib.dll.pscf_batch_destroy(self.h)
+            self.h = 0
+
+    def add(self, scfsys, omega):
+        s = scfsys.system
+        cell = np.ascontiguousarray(s.cell_param, dtype=np.float64)
+        om = np.ascontiguousarray(np.asarray(omega, dtype=np.float64).T)
+        phs = np.ascontiguousarray(list(s.phi_solvent) or [0.0], dtype=np.float64)
+        idx = ctypes.c_int(0)
+        self.lib.call("pscf_batch_add", self.h, scfsys.h, s.crystal_system.encode(), len(cell),
+                      dptr(cell), dptr(om), dptr(scfsys.phi_chain), dptr(scfsys.mu_chain),
+                      dptr(phs), ctypes.byref(idx))
+        self.systems.append(scfsys)
+        return idx.value
+
+    def solve(self):
+        it = ctypes.c_int(0)
+        self.lib.call("pscf_batch_solve", self.h, ctypes.byref(it))
+        return it.value
+
+    def result(self, i):
+        S = self.systems[i]
+        nm, ns = S.system.N_monomer, S.N_star
+        om = np.zeros(nm * ns)
+        rho = np.zeros(nm * ns)
+        cell = np.zeros(len(S.system.cell_param))
+        st = np.zeros(len(S.system.cell_param))
+        th = np.zeros(4)
+        itr = ctypes.c_int(0)
+        status = ctypes.c_int(0)
+        self.lib.call("pscf_batch_result", self.h, int(i), dptr(om), dptr(rho), dptr(cell),
+                      dptr(st), dptr(th), ctypes.byref(itr), ctypes.byref(status))
+        return BatchResult(omega=om.reshape(ns, nm).T.copy(), rho=rho.reshape(ns, nm).T.copy(),
+                           cell_param=cell, stress=st, f_helmholtz=th[0], pressure=th[1],
+                           error=th[2], mu_chain=th[3], itr=itr.value,
+                           converge=status.value == 1, status=status.value)
+
+
+def make_system(system, tables, lib=None):
+    """SCFSystem for `system` with star tables `tables` at the cell of system.cell_param."""
+    from .iterate import unit_cell
+    from ._lib import lib as _default_lib
+    lib = lib or _default_lib()
+    _, G, _ = unit_cell(lib, system.dim, system.crystal_system, system.cell_param)
+    return scf.SCFSystem(system, scf.Basis.from_tables(tables), G, lib=lib)

@@ -28,6 +28,7 @@ struct StepTask {
   int ns;
   int dir;                      // +1 forward sweep, -1 backward sweep
   int init_one;                 // 1: the first point is the constant 1 (and is written)
+  int skip;                     // 1: inactive (converged instance of a batch): every kernel returns
   int nblk;
   int bnd[MAXBLK + 1];
   const double* e1[MAXBLK];     // exp(-ds/2 w)   (M)
@@ -66,7 +67,7 @@ __global__ void __launch_bounds__(NT) k_step_a(const StepTask* __restrict__ task
   __shared__ double2 H[PL::HSIZE];
   const int tid = threadIdx.x, z = blockIdx.x;
   const StepTask& t = tasks[blockIdx.y];
-  if (j > t.ns) return;
+  if (t.skip || j > t.ns) return;
   const long M = (long)N1 * N2 * N3;
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(NT) k_step_c(const StepTask* __restrict__ task
   __shared__ double2 H[PL::HSIZE];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
-  if (j >= t.ns) return;
+  if (t.skip || j >= t.ns) return;
   const long M = (long)N1 * N2 * N3;
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
@@ -203,7 +204,7 @@ __global__ void __launch_bounds__(NT) k_step_z(const StepTask* __restrict__ task
   __shared__ double2 T[N1H * N3];
   const int tid = threadIdx.x, ky = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
-  if (j >= t.ns) return;
+  if (t.skip || j >= t.ns) return;
   const int b = task_block(t, j);
   double2* arr;
   const double* kf;

@@ -299,7 +299,8 @@ struct System {
   DevBuf<double2> abc;     // 3 * mk per task
   DevBuf<double> q1;       // m per task
   DevBuf<StepTask> tasks;
-  std::vector<double> h_Q, h_Qs;
+  std::vector<StepTask> h_tasks;
+  std::vector<double> h_Q, h_Qs, h_chem, h_dots;
   bool have_density = false;
   // contour sweeps of one density() captured once as a HIP graph
   int ntask = 0, max_ns = 0;
@@ -496,21 +497,22 @@ void apply_cell(System& S) {
 }
 
 // star coefficients (device, (nf, nstar)) -> real grids (nf arrays)
-void basis_to_grid(System& S, const double* d_basis, int nf, double* d_grid) {
+void basis_to_grid(System& S, const double* d_basis, int nf, double* d_grid, hipStream_t s) {
   const Grid& g = S.g;
   S.kwork.alloc((size_t)std::max(nf, 2) * g.mk);
-  hipLaunchKernelGGL(k_basis_to_kgrid, dim3(grid_blocks(g.mk * nf)), dim3(NTA), 0, S.st.s, d_basis, nf,
+  hipLaunchKernelGGL(k_basis_to_kgrid, dim3(grid_blocks(g.mk * nf)), dim3(NTA), 0, s, d_basis, nf,
                      S.nstar, S.kwave.p, S.coeff.p, S.star_of_wave.p, S.star_invert.p, g.mk,
                      S.kwork.p, 0);
-  g.c2r(S.kwork.p, d_grid, nf, 1.0, S.st.s);
+  g.c2r(S.kwork.p, d_grid, nf, 1.0, s);
 }
 
 // real grids (nf arrays, stride m) -> star coefficients (nf, nstar) * scale
-void grid_to_basis(System& S, const double* d_grid, int nf, double scale, double* d_basis) {
+void grid_to_basis(System& S, const double* d_grid, int nf, double scale, double* d_basis,
+                   hipStream_t s) {
   const Grid& g = S.g;
   S.kwork.alloc((size_t)std::max(nf, 2) * g.mk);
-  g.r2c(d_grid, S.kwork.p, nf, S.st.s);
-  hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks((long)S.nstar * nf)), dim3(NTA), 0, S.st.s,
+  g.r2c(d_grid, S.kwork.p, nf, s);
+  hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks((long)S.nstar * nf)), dim3(NTA), 0, s,
                      S.kwork.p, g.mk, nf, S.nstar, S.sidx.p, S.sconj.p, S.scoef.p, S.star_invert.p,
                      scale, d_basis, 0);
 }
@@ -582,6 +584,7 @@ void startup(System& S, double chain_step) {
     HIPCHK(hipMemcpyAsync(S.tasks.p, ht.data(), sizeof(StepTask) * ntask, hipMemcpyHostToDevice,
                           S.st.s));
   S.st.sync();
+  S.h_tasks = ht;
   S.ntask = ntask;
   S.max_ns = max_ns;
   S.drop_graph();
@@ -589,16 +592,17 @@ void startup(System& S, double chain_step) {
   S.have_density = false;
 }
 
-void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
-             double* mu_c, double* phi_s, double* mu_s) {
+// ---- density() in phases, each enqueued on stream s (src/scf_mod.fp.f90:144-229)
+
+// omega stars -> grid (no 1/M, :182-185), propagator factors per block
+// (make_propg), chemistry scalars for the device
+void density_pre(System& S, const double* omega, const double* phi_c, const double* mu_c,
+                 const double* phi_s, const double* mu_s, hipStream_t s) {
   REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
   const Grid& g = S.g;
-  hipStream_t s = S.st.s;
   const size_t nst = (size_t)S.nmon * S.nstar;
   S.omega_in.upload(omega, nst, s);
-  // omega stars -> grid, no 1/M (src/scf_mod.fp.f90:182-185)
-  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p);
-  // propagator factors per block (make_propg, src/step_mod.f90:109-129)
+  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p, s);
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     for (int b = 0; b < ch.nblk; ++b) {
@@ -608,51 +612,56 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
                          g.m, f, f + g.m, f + 2 * g.m, f + 2 * g.m + g.mk);
     }
   }
-  // chemistry scalars read on the device (phi, mu per chain and solvent)
-  {
-    std::vector<double> cd(2 * (S.nchain + S.nsolv) + 2, 0.0);
-    for (int c = 0; c < S.nchain; ++c) {
-      cd[2 * c] = phi_c ? phi_c[c] : 0.0;
-      cd[2 * c + 1] = mu_c ? mu_c[c] : 0.0;
-    }
-    for (int i = 0; i < S.nsolv; ++i) {
-      cd[2 * (S.nchain + i)] = phi_s ? phi_s[i] : 0.0;
-      cd[2 * (S.nchain + i) + 1] = mu_s ? mu_s[i] : 0.0;
-    }
-    HIPCHK(hipMemcpyAsync(S.chem_dev.p, cd.data(), sizeof(double) * cd.size(),
-                          hipMemcpyHostToDevice, s));
+  S.h_chem.assign(2 * (S.nchain + S.nsolv) + 2, 0.0);
+  for (int c = 0; c < S.nchain; ++c) {
+    S.h_chem[2 * c] = phi_c ? phi_c[c] : 0.0;
+    S.h_chem[2 * c + 1] = mu_c ? mu_c[c] : 0.0;
   }
-  // both sweeps of every chain: 4 launches per contour step
+  for (int i = 0; i < S.nsolv; ++i) {
+    S.h_chem[2 * (S.nchain + i)] = phi_s ? phi_s[i] : 0.0;
+    S.h_chem[2 * (S.nchain + i) + 1] = mu_s ? mu_s[i] : 0.0;
+  }
+  HIPCHK(hipMemcpyAsync(S.chem_dev.p, S.h_chem.data(), sizeof(double) * S.h_chem.size(),
+                        hipMemcpyHostToDevice, s));
+}
+
+// both contour sweeps of every chain: 4 launches per contour step
+void density_sweeps(System& S, hipStream_t s) {
+  const Grid& g = S.g;
   const int max_ns = S.max_ns;
   if (S.profile == 2) {
     ensure_events(S, (size_t)4 * max_ns + 2);
     run_sweeps(g, S.tasks.p, S.ntask, max_ns, s, S.ev.data());
-  } else {
-    if (S.profile == 1) {
-      ensure_events(S, 2);
-      HIPCHK(hipEventRecord(S.ev[0], s));
-    }
-    if (S.use_graph) {
-      if (!S.graph_exec) {
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
-        HIPCHK(hipStreamEndCapture(s, &S.graph));
-        HIPCHK(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0));
-      }
-      HIPCHK(hipGraphLaunch(S.graph_exec, s));
-    } else {
-      run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
-    }
-    if (S.profile == 1) HIPCHK(hipEventRecord(S.ev[1], s));
+    return;
   }
-  // Q = mean of qf at the chain end (src/scf_mod.fp.f90:359-360)
+  if (S.profile == 1) {
+    ensure_events(S, 2);
+    HIPCHK(hipEventRecord(S.ev[0], s));
+  }
+  if (S.use_graph) {
+    if (!S.graph_exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
+      HIPCHK(hipStreamEndCapture(s, &S.graph));
+      HIPCHK(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(S.graph_exec, s));
+  } else {
+    run_sweeps(g, S.tasks.p, S.ntask, max_ns, s);
+  }
+  if (S.profile == 1) HIPCHK(hipEventRecord(S.ev[1], s));
+}
+
+// Q (:359-360), solvents (:193-200), Simpson rho (:362-424, 212-218),
+// rho grid -> stars / M (:221-225); copies Q and rho stars to the host
+void density_post(System& S, double* rho, hipStream_t s) {
+  const Grid& g = S.g;
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     dev_sum(ch.qf.p + (size_t)ch.ns * g.m, g.m, 1.0 / (double)g.m, S.part.p, S.qdev.p + c, s);
   }
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)S.nmon * g.m)), dim3(NTA), 0, s, S.rho_grid.p,
                      (long)S.nmon * g.m, 0.0);
-  // solvents (src/scf_mod.fp.f90:193-200, 251-295)
   for (int i = 0; i < S.nsolv; ++i) {
     const double* w = S.omega_grid.p + (size_t)S.solv_mon[i] * g.m;
     const int nb = 512;
@@ -663,7 +672,6 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
                        S.qdev.p + S.nchain + i, S.ensemble, S.chem_dev.p + 2 * (S.nchain + i), g.m,
                        S.rho_grid.p + (size_t)S.solv_mon[i] * g.m);
   }
-  // Simpson rho per chain (src/scf_mod.fp.f90:362-424, 212-218)
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     SimpsonArgs a;
@@ -679,21 +687,24 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
     hipLaunchKernelGGL(k_simpson, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a,
                        S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
   }
-  // rho grid -> stars, / M (src/scf_mod.fp.f90:221-225)
+  const size_t nst = (size_t)S.nmon * S.nstar;
   S.rho_out.alloc(nst);
-  grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p);
+  grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p, s);
   HIPCHK(hipGetLastError());
   S.h_Q.assign(S.nchain + S.nsolv, 0.0);
   HIPCHK(hipMemcpyAsync(S.h_Q.data(), S.qdev.p, sizeof(double) * (S.nchain + S.nsolv),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
-  S.st.sync();
-  if (S.profile) collect_profile(S, max_ns);
+}
+
+// after the stream is synchronised: outputs and the grand-ensemble update (:202-209)
+void density_finish(System& S, double* qout, double* qsolv, double* phi_c, double* mu_c,
+                    double* phi_s, double* mu_s) {
+  if (S.profile) collect_profile(S, S.max_ns);
   for (int c = 0; c < S.nchain; ++c)
     if (qout) qout[c] = S.h_Q[c];
   for (int i = 0; i < S.nsolv; ++i)
     if (qsolv) qsolv[i] = S.h_Q[S.nchain + i];
-  // grand ensemble: phi from mu (src/scf_mod.fp.f90:202-209, 768-820)
   if (S.ensemble == 1) {
     for (int c = 0; c < S.nchain; ++c)
       if (phi_c && mu_c) phi_c[c] = S.h_Q[c] * std::exp(mu_c[c]);
@@ -703,24 +714,33 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
   S.have_density = true;
 }
 
-// scf_stress (src/scf_mod.fp.f90:451-550)
-void stress(System& S, int ncp, const double* dgsq, const double* phi_c, const double* mu_c,
-            double* out) {
+void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
+             double* mu_c, double* phi_s, double* mu_s) {
+  hipStream_t s = S.st.s;
+  density_pre(S, omega, phi_c, mu_c, phi_s, mu_s, s);
+  density_sweeps(S, s);
+  density_post(S, rho, s);
+  S.st.sync();
+  density_finish(S, qout, qsolv, phi_c, mu_c, phi_s, mu_s);
+}
+
+// ---- scf_stress (src/scf_mod.fp.f90:451-550)
+
+// per contour point p of every chain: sum_star qr_basis * dGsq * qf_basis
+// (:505-528) on the device, copied to the host (S.h_dots)
+void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
   REQUIRE(S.have_density, "scf_stress requires a previous density()");
   const Grid& g = S.g;
-  hipStream_t s = S.st.s;
   S.dgsq.upload(dgsq, (size_t)S.nstar * ncp, s);
-  const double rnodes = (double)g.m;
-  const double normal = rnodes * rnodes * 3.0 * 6.0;        // :485-488
   const int chunk = 32;
   S.sbasis.alloc((size_t)2 * chunk * S.nstar);
   S.skbuf.alloc((size_t)chunk * g.mk);
-  S.sdots.alloc((size_t)chunk * ncp);
-  for (int k = 0; k < ncp; ++k) out[k] = 0.0;
+  size_t npts = 0;
+  for (int sp = 0; sp < S.nchain; ++sp) npts += (size_t)S.chains[sp]->ns + 1;
+  S.sdots.alloc(npts * ncp);
+  size_t off = 0;
   for (int sp = 0; sp < S.nchain; ++sp) {
     ChainDev& ch = *S.chains[sp];
-    // dot products for every contour point 0..ns
-    std::vector<double> alld((size_t)(ch.ns + 1) * ncp);
     for (int p0 = 0; p0 <= ch.ns; p0 += chunk) {
       const int np = std::min(chunk, ch.ns + 1 - p0);
       for (int d = 0; d < 2; ++d) {
@@ -731,12 +751,26 @@ void stress(System& S, int ncp, const double* dgsq, const double* phi_c, const d
                            S.star_invert.p, 1.0, S.sbasis.p + (size_t)d * chunk * S.nstar, 0);
       }
       hipLaunchKernelGGL(k_stress_dots, dim3(np), dim3(NTA), 0, s, S.sbasis.p,
-                         S.sbasis.p + (size_t)chunk * S.nstar, np, S.nstar, S.dgsq.p, ncp, S.sdots.p);
-      HIPCHK(hipMemcpyAsync(alld.data() + (size_t)p0 * ncp, S.sdots.p, sizeof(double) * np * ncp,
-                            hipMemcpyDeviceToHost, s));
+                         S.sbasis.p + (size_t)chunk * S.nstar, np, S.nstar, S.dgsq.p, ncp,
+                         S.sdots.p + (off + p0) * ncp);
     }
-    HIPCHK(hipGetLastError());
-    S.st.sync();
+    off += (size_t)ch.ns + 1;
+  }
+  HIPCHK(hipGetLastError());
+  S.h_dots.resize(npts * ncp);
+  HIPCHK(hipMemcpyAsync(S.h_dots.data(), S.sdots.p, sizeof(double) * npts * ncp,
+                        hipMemcpyDeviceToHost, s));
+}
+
+// after synchronisation: Simpson weights and the sum over contour points in
+// the reference's order (:497-544)
+void stress_finish(System& S, int ncp, const double* phi_c, const double* mu_c, double* out) {
+  const double rnodes = (double)S.g.m;
+  const double normal = rnodes * rnodes * 3.0 * 6.0;        // :485-488
+  for (int k = 0; k < ncp; ++k) out[k] = 0.0;
+  size_t off = 0;
+  for (int sp = 0; sp < S.nchain; ++sp) {
+    ChainDev& ch = *S.chains[sp];
     std::vector<double> dQ(ncp, 0.0);
     for (int b = 0; b < ch.nblk; ++b) {
       const double bk = S.kuhn[ch.mon[b]];
@@ -746,18 +780,272 @@ void stress(System& S, int ncp, const double* dgsq, const double* phi_c, const d
         double ds = ds0;
         if (i != ibgn && i != iend) ds = (i % 2 == 0) ? 4.0 * ds : 2.0 * ds;
         for (int beta = 0; beta < ncp; ++beta) {
-          double inc = alld[(size_t)(i - 1) * ncp + beta];
+          double inc = S.h_dots[(off + (size_t)(i - 1)) * ncp + beta];
           inc = inc * (bk * bk) * ds / normal;
           dQ[beta] = dQ[beta] - inc;
         }
       }
     }
+    off += (size_t)ch.ns + 1;
     const double Q = S.h_Q[sp];
     for (int beta = 0; beta < ncp; ++beta) {
       if (S.ensemble == 0)
         out[beta] = out[beta] - (dQ[beta] / Q) * phi_c[sp] / ch.length;
       else
         out[beta] = out[beta] - (dQ[beta] / Q) * std::exp(mu_c[sp]) * Q / ch.length;
+    }
+  }
+}
+
+void stress(System& S, int ncp, const double* dgsq, const double* phi_c, const double* mu_c,
+            double* out) {
+  stress_launch(S, ncp, dgsq, S.st.s);
+  S.st.sync();
+  stress_finish(S, ncp, phi_c, mu_c, out);
+}
+
+
+// =========================================================== batched solves
+// Many independent systems iterated together (iterate_AM's control flow,
+// src/iterate_mod.fp.f90:1042-1490, per instance).  Systems with the same
+// grid form a group whose contour sweeps run in one launch set per step
+// (all instances' sweep tasks side by side), captured once as a HIP graph;
+// converged instances stay in the graph with their tasks marked skip.
+
+// free energy and pressure (src/scf_mod.fp.f90:834-886)
+void thermo(const System& S, const double* rho, const double* omega, const double* phi_c,
+            const double* mu_c, const double* phi_s, const double* mu_s, double* fo, double* po) {
+  const int nm = S.nmon;
+  double f = 0.0;
+  for (int i = 0; i < S.nchain; ++i) {
+    double L = 0.0;
+    for (const auto& b : S.blocks[i]) L = L + b.second;
+    if (phi_c[i] > 1.0e-8f) f = f + phi_c[i] * (mu_c[i] - 1.0) / L;
+  }
+  for (int i = 0; i < S.nsolv; ++i)
+    if (phi_s[i] > 1.0e-8f) f = f + phi_s[i] * (mu_s[i] - 1.0) / S.solv_size[i];
+  for (int i = 0; i < S.nstar; ++i)
+    for (int a = 0; a < nm; ++a) {
+      for (int b = a + 1; b < nm; ++b)
+        f = f + rho[a + (size_t)nm * i] * S.chi[a + (size_t)nm * b] * rho[b + (size_t)nm * i];
+      f = f - omega[a + (size_t)nm * i] * rho[a + (size_t)nm * i];
+    }
+  double p = -f;
+  for (int i = 0; i < S.nchain; ++i) {
+    double L = 0.0;
+    for (const auto& b : S.blocks[i]) L = L + b.second;
+    if (phi_c[i] > 1.0e-8f) p = p + mu_c[i] * phi_c[i] / L;
+  }
+  for (int i = 0; i < S.nsolv; ++i)
+    if (phi_s[i] > 1.0e-8f) p = p + mu_s[i] * phi_s[i] / S.solv_size[i];
+  *fo = f;
+  *po = p;
+}
+
+struct BatchInst {
+  System* S = nullptr;
+  int64_t am = 0;
+  std::string cs;
+  int ncp = 0;
+  std::vector<double> cell, omega, rho, dgsq, stress, phi_c, mu_c, phi_s, mu_s, q, qs;
+  double f = 0.0, p = 0.0, err = 0.0;
+  int itr = 0, status = -1;
+  bool active = true;
+};
+
+struct Group {
+  int n1 = 0, n2 = 0, n3 = 0;
+  std::vector<int> members;
+  std::vector<StepTask> h_tasks;
+  std::vector<int> task_owner;
+  DevBuf<StepTask> tasks;
+  int max_ns = 0;
+  Stream st;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~Group() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+struct Batch {
+  int nhist = 20, domain = 1, max_itr = 1000;
+  double error_max = 1e-8;
+  std::vector<BatchInst> inst;
+  std::vector<std::unique_ptr<Group>> groups;
+  int iterations = 0;
+  ~Batch() {
+    for (auto& b : inst)
+      if (b.am) pscf_am_destroy(b.am);
+  }
+};
+
+void dgsq_of(BatchInst& B) {
+  System& S = *B.S;
+  const int dim = S.g.dim;
+  std::vector<double> dGG((size_t)dim * dim * 6);
+  if (pscf_unit_cell(dim, B.cs.c_str(), B.ncp, B.cell.data(), nullptr, nullptr, dGG.data()))
+    throw Fail{std::string("unit cell: ") + pscf_am_last_error()};
+  B.dgsq.assign((size_t)S.nstar * B.ncp, 0.0);
+  for (int k = 0; k < B.ncp; ++k)
+    if (pscf_make_dgsq(dim, S.nstar, S.h_wave_of_star.data(), dGG.data() + (size_t)dim * dim * k,
+                       B.dgsq.data() + (size_t)S.nstar * k))
+      throw Fail{std::string("make_dGsq: ") + pscf_am_last_error()};
+}
+
+void build_groups(Batch& Bt) {
+  Bt.groups.clear();
+  for (int i = 0; i < (int)Bt.inst.size(); ++i) {
+    System& S = *Bt.inst[i].S;
+    Group* G = nullptr;
+    for (auto& g : Bt.groups)
+      if (g->n1 == S.g.n1 && g->n2 == S.g.n2 && g->n3 == S.g.n3) G = g.get();
+    if (!G) {
+      Bt.groups.push_back(std::make_unique<Group>());
+      G = Bt.groups.back().get();
+      G->n1 = S.g.n1;
+      G->n2 = S.g.n2;
+      G->n3 = S.g.n3;
+    }
+    G->members.push_back(i);
+    for (const auto& t : S.h_tasks) {
+      G->h_tasks.push_back(t);
+      G->task_owner.push_back(i);
+    }
+    G->max_ns = std::max(G->max_ns, S.max_ns);
+  }
+  for (auto& G : Bt.groups) G->tasks.alloc(G->h_tasks.size());
+}
+
+// one batched density() (+ scf_stress when domain) over the active instances
+void batch_density(Batch& Bt, bool with_stress) {
+  for (auto& Gp : Bt.groups) {
+    Group& G = *Gp;
+    hipStream_t s = G.st.s;
+    for (size_t k = 0; k < G.h_tasks.size(); ++k) G.h_tasks[k].skip = Bt.inst[G.task_owner[k]].active ? 0 : 1;
+    HIPCHK(hipMemcpyAsync(G.tasks.p, G.h_tasks.data(), sizeof(StepTask) * G.h_tasks.size(),
+                          hipMemcpyHostToDevice, s));
+    for (int i : G.members) {
+      BatchInst& B = Bt.inst[i];
+      if (!B.active) continue;
+      density_pre(*B.S, B.omega.data(), B.phi_c.data(), B.mu_c.data(), B.phi_s.data(),
+                  B.mu_s.data(), s);
+    }
+    const Grid& g = Bt.inst[G.members[0]].S->g;
+    if (!G.exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_sweeps(g, G.tasks.p, (int)G.h_tasks.size(), G.max_ns, s);
+      HIPCHK(hipStreamEndCapture(s, &G.graph));
+      HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(G.exec, s));
+    for (int i : G.members) {
+      BatchInst& B = Bt.inst[i];
+      if (!B.active) continue;
+      density_post(*B.S, B.rho.data(), s);
+      B.S->have_density = true;
+      if (with_stress) stress_launch(*B.S, B.ncp, B.dgsq.data(), s);
+    }
+  }
+  for (auto& Gp : Bt.groups) Gp->st.sync();
+  for (auto& B : Bt.inst) {
+    if (!B.active) continue;
+    System& S = *B.S;
+    density_finish(S, B.q.data(), B.qs.data(), B.phi_c.data(), B.mu_c.data(), B.phi_s.data(),
+                   B.mu_s.data());
+    if (with_stress) stress_finish(S, B.ncp, B.phi_c.data(), B.mu_c.data(), B.stress.data());
+  }
+}
+
+void mu_phi(BatchInst& B) {
+  System& S = *B.S;
+  for (int i = 0; i < S.nchain; ++i) {
+    if (S.ensemble == 0) B.mu_c[i] = std::log(B.phi_c[i] / B.q[i]);
+    else B.phi_c[i] = B.q[i] * std::exp(B.mu_c[i]);
+  }
+  for (int i = 0; i < S.nsolv; ++i) {
+    if (S.ensemble == 0) B.mu_s[i] = std::log(B.phi_s[i] / B.qs[i]);
+    else B.phi_s[i] = B.qs[i] * std::exp(B.mu_s[i]);
+  }
+}
+
+void batch_solve(Batch& Bt) {
+  REQUIRE(!Bt.inst.empty(), "empty batch");
+  if (Bt.groups.empty()) build_groups(Bt);
+  for (auto& B : Bt.inst) {
+    B.active = true;
+    B.itr = 0;
+    B.status = -1;
+    if (B.am) pscf_am_destroy(B.am);
+    if (pscf_am_create(B.S->nstar, B.S->nmon, B.ncp, Bt.nhist, Bt.domain, &B.am))
+      throw Fail{std::string("am_create: ") + pscf_am_last_error()};
+  }
+  // initial density and mu (src/iterate_mod.fp.f90:1123-1128), dGsq, omega(:,1)
+  batch_density(Bt, false);
+  for (auto& B : Bt.inst) {
+    mu_phi(B);
+    dgsq_of(B);
+    System& S = *B.S;
+    if (S.ensemble == 0) {
+      std::vector<double> phi_mon(S.nmon, 0.0);
+      for (int i = 0; i < S.nchain; ++i) {
+        double L = 0.0;
+        for (const auto& b : S.blocks[i]) L = L + b.second;
+        for (const auto& b : S.blocks[i]) phi_mon[b.first] = phi_mon[b.first] + B.phi_c[i] * b.second / L;
+      }
+      for (int i = 0; i < S.nsolv; ++i) phi_mon[S.solv_mon[i]] = phi_mon[S.solv_mon[i]] + B.phi_s[i];
+      for (int a = 0; a < S.nmon; ++a) {
+        B.omega[a] = 0.0;
+        for (int b = 0; b < S.nmon; ++b) B.omega[a] = B.omega[a] + S.chi[a + (size_t)S.nmon * b] * phi_mon[b];
+      }
+    }
+    if (pscf_am_start(B.am, B.omega.data(), B.cell.data()))
+      throw Fail{std::string("am_start: ") + pscf_am_last_error()};
+  }
+  Bt.iterations = 0;
+  for (;;) {
+    bool any = false;
+    for (auto& B : Bt.inst) any = any || B.active;
+    if (!any) break;
+    ++Bt.iterations;
+    batch_density(Bt, Bt.domain != 0);
+    for (auto& B : Bt.inst) {
+      if (!B.active) continue;
+      System& S = *B.S;
+      ++B.itr;
+      mu_phi(B);
+      thermo(S, B.rho.data(), B.omega.data(), B.phi_c.data(), B.mu_c.data(), B.phi_s.data(),
+             B.mu_s.data(), &B.f, &B.p);
+      int st = 0;
+      if (pscf_am_iterate(B.am, B.omega.data(), B.rho.data(), S.chi.data(), B.stress.data(),
+                          B.cell.data(), Bt.error_max, Bt.max_itr, &B.err, &st))
+        throw Fail{std::string("am_iterate: ") + pscf_am_last_error()};
+      if (st != 0) {
+        B.status = st;
+        B.active = false;
+        continue;
+      }
+      if (Bt.domain) {
+        const int dim = S.g.dim;
+        std::vector<double> G((size_t)dim * dim);
+        if (pscf_unit_cell(dim, B.cs.c_str(), B.ncp, B.cell.data(), nullptr, G.data(), nullptr))
+          throw Fail{std::string("unit cell: ") + pscf_am_last_error()};
+        for (int i = 0; i < 9; ++i) S.gb[i] = 0.0;
+        for (int i = 0; i < dim; ++i)
+          for (int jj = 0; jj < dim; ++jj) S.gb[i * 3 + jj] = G[i + dim * jj];
+        apply_cell(S);
+        dgsq_of(B);
+      }
+    }
+    for (auto& B : Bt.inst) B.S->st.sync();   // cell updates run on each system's stream
+  }
+  // fixed cell: residual stress of the final fields (:1473)
+  if (!Bt.domain) {
+    for (auto& B : Bt.inst) {
+      stress_launch(*B.S, B.ncp, B.dgsq.data(), B.S->st.s);
+      B.S->st.sync();
+      stress_finish(*B.S, B.ncp, B.phi_c.data(), B.mu_c.data(), B.stress.data());
     }
   }
 }
@@ -1279,6 +1567,81 @@ int pscf_kgrid_to_basis(int64_t sys, const double* kgrid, double* basis, int no_
     HIPCHK(hipMemcpyAsync(basis, S.basis_tmp.p, sizeof(double) * S.nstar, hipMemcpyDeviceToHost,
                           S.st.s));
     S.st.sync();
+  });
+}
+
+// ------------------------------------------------------------ batched solves
+int pscf_batch_create(int n_hist, int domain, double error_max, int max_itr, int64_t* batch) {
+  return guarded([&] {
+    auto* b = new Batch;
+    b->nhist = n_hist;
+    b->domain = domain;
+    b->error_max = error_max;
+    b->max_itr = max_itr;
+    *batch = reinterpret_cast<int64_t>(b);
+  });
+}
+
+int pscf_batch_destroy(int64_t batch) {
+  return guarded([&] { delete from_handle<Batch>(batch, "batch"); });
+}
+
+int pscf_batch_add(int64_t batch, int64_t sys, const char* crystal_system, int n_cell_param,
+                   const double* cell_param, const double* omega, const double* phi_chain,
+                   const double* mu_chain, const double* phi_solvent, int* index) {
+  return guarded([&] {
+    Batch& Bt = *from_handle<Batch>(batch, "batch");
+    System* S = from_handle<System>(sys, "system");
+    REQUIRE(S->started && S->have_basis && S->have_cell, "system not ready (basis, cell, startup)");
+    REQUIRE(S->chi.size() == (size_t)S->nmon * S->nmon, "chi not set");
+    BatchInst B;
+    B.S = S;
+    B.cs = crystal_system;
+    B.ncp = n_cell_param;
+    B.cell.assign(cell_param, cell_param + n_cell_param);
+    B.omega.assign(omega, omega + (size_t)S->nmon * S->nstar);
+    B.rho.assign((size_t)S->nmon * S->nstar, 0.0);
+    B.stress.assign(n_cell_param, 0.0);
+    B.phi_c.assign(phi_chain, phi_chain + S->nchain);
+    B.mu_c.assign(S->nchain, 0.0);
+    if (mu_chain) B.mu_c.assign(mu_chain, mu_chain + S->nchain);
+    B.phi_s.assign(std::max(S->nsolv, 1), 0.0);
+    B.mu_s.assign(std::max(S->nsolv, 1), 0.0);
+    for (int i = 0; i < S->nsolv; ++i) B.phi_s[i] = phi_solvent[i];
+    B.q.assign(std::max(S->nchain, 1), 0.0);
+    B.qs.assign(std::max(S->nsolv, 1), 0.0);
+    Bt.inst.push_back(std::move(B));
+    Bt.groups.clear();
+    *index = (int)Bt.inst.size() - 1;
+  });
+}
+
+int pscf_batch_solve(int64_t batch, int* iterations) {
+  return guarded([&] {
+    Batch& Bt = *from_handle<Batch>(batch, "batch");
+    batch_solve(Bt);
+    if (iterations) *iterations = Bt.iterations;
+  });
+}
+
+int pscf_batch_result(int64_t batch, int index, double* omega, double* rho, double* cell_param,
+                      double* stress, double* thermo_out, int* itr, int* status) {
+  return guarded([&] {
+    Batch& Bt = *from_handle<Batch>(batch, "batch");
+    REQUIRE(index >= 0 && index < (int)Bt.inst.size(), "batch index out of range");
+    const BatchInst& B = Bt.inst[index];
+    if (omega) std::copy(B.omega.begin(), B.omega.end(), omega);
+    if (rho) std::copy(B.rho.begin(), B.rho.end(), rho);
+    if (cell_param) std::copy(B.cell.begin(), B.cell.end(), cell_param);
+    if (stress) std::copy(B.stress.begin(), B.stress.end(), stress);
+    if (thermo_out) {
+      thermo_out[0] = B.f;
+      thermo_out[1] = B.p;
+      thermo_out[2] = B.err;
+      thermo_out[3] = B.S->nchain > 0 ? B.mu_c[0] : 0.0;
+    }
+    if (itr) *itr = B.itr;
+    if (status) *status = B.status;
   });
 }
 

@@ -1,0 +1,90 @@
+"""Batched solves (pscf_amd.batch) against the single-instance path.
+
+Each instance of a batch runs the same kernels and the same Anderson
+arithmetic as a lone solve, so iteration counts and converged fields must
+agree with AndersonMixing on that instance alone; instances on different
+grids form separate launch groups.  The lamellar instances also reproduce the
+reference's published known answer at chi N = 15
+(tools/python/tests/outfile/in:70-77).
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, get_lib, load_golden
+
+sys.path.insert(0, GOLDEN)
+from make_golden import cases  # noqa: E402
+
+from pscf_amd import batch, iterate  # noqa: E402
+
+
+def _instances(chis, step):
+    g = load_golden("lam32")
+    base = cases()["lam32"][0]
+    start = load_golden("lam32_start")["omega"]
+    out = []
+    for c in chis:
+        s = copy.deepcopy(base)
+        s.chi = np.array([[0.0, c], [c, 0.0]])
+        s.chain_step = step
+        out.append((s, g, start))
+    return out
+
+
+def _hex24():
+    g = load_golden("hex24")
+    s = cases()["hex24"][0]
+    om = g["omega"].copy()
+    om[:, 1:] *= 0.2
+    return s, g, om
+
+
+def _bcc8():
+    g = load_golden("bcc8")
+    s = cases()["bcc8"][0]
+    om = g["omega"].copy()
+    om[:, 1:] *= 0.2
+    return s, g, om
+
+
+@pytest.mark.parametrize("domain", [False, True])
+def test_batch_equals_single_instance(backend, domain):
+    lib = get_lib(backend)
+    if backend == "emu":      # two cheap lamellar instances (20 contour steps), one group
+        insts = _instances([15.000000011, 16.5], 0.05)
+        emax, max_itr = 1e-6, 60
+    else:                     # lamellar, BCC 8^3 and hexagonal 24^2: three launch groups
+        insts = _instances([15.000000011, 16.5], 0.01) + [_bcc8(), _hex24()]
+        emax, max_itr = 1e-10, 300
+    B = batch.BatchSolver(n_hist=10, domain=domain, error_max=emax, max_itr=max_itr, lib=lib)
+    singles = []
+    for s, g, om in insts:
+        B.add(batch.make_system(s, g, lib=lib), om)
+        S1 = batch.make_system(s, g, lib=lib)
+        singles.append(iterate.AndersonMixing(S1, n_hist=10, domain=domain).run(
+            om, error_max=emax, max_itr=max_itr))
+    B.solve()
+    for i, r1 in enumerate(singles):
+        r = B.result(i)
+        assert r.converge == r1.converge and r.itr == r1.itr, (i, r.itr, r1.itr)
+        assert abs(r.f_helmholtz - r1.f_helmholtz) < 1e-12 * abs(r1.f_helmholtz), i
+        np.testing.assert_allclose(r.omega, r1.omega, rtol=0, atol=1e-11 * np.abs(r1.omega).max())
+        np.testing.assert_allclose(r.cell_param, r1.cell_param, rtol=1e-13)
+
+
+@pytest.mark.gpu
+def test_batch_known_answer_gpu():
+    lib = get_lib("gpu")
+    insts = _instances([15.000000011], 0.01)
+    B = batch.BatchSolver(n_hist=10, domain=False, error_max=1e-10, max_itr=300, lib=lib)
+    for s, g, om in insts:
+        B.add(batch.make_system(s, g, lib=lib), om)
+    B.solve()
+    r = B.result(0)
+    assert r.converge
+    assert abs(r.f_helmholtz / 2.4293253659 - 1) < 2e-10
+    assert abs(r.pressure / 3.0121269977 - 1) < 2e-10
+    assert abs(r.mu_chain / 5.4414523636 - 1) < 2e-10
