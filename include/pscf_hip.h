@@ -226,6 +226,36 @@ int pscf_batch_solve(int64_t batch, int* iterations);
 int pscf_batch_result(int64_t batch, int index, double* omega, double* rho, double* cell_param,
                       double* stress, double* thermo, int* itr, int* status);
 
+/* ------------------------------------------------- reverse mode (adjoint)
+ * Derivatives of density() for gradient-based design and fitting.  No
+ * reference counterpart (the reference has no derivative code); the adjoint
+ * sweeps reuse the contour-step kernels of step_exp (src/step_mod.f90:146-176)
+ * and the Simpson weights of chain_density (src/scf_mod.fp.f90:362-424).
+ * Grid arrays here are DEVICE pointers, laid out (M, N_monomer) with x
+ * fastest, as omega_grid/rho_grid in src/grid_mod.f90:44-45. */
+/* density() from omega grids (no star projection): omega_grid in, rho_grid
+ * out (before the star projection of src/scf_mod.fp.f90:221-225), qout and
+ * the grand-ensemble updates as pscf_density. */
+int pscf_density_grid_dev(int64_t sys, const double* omega_grid, double* rho_grid, double* qout,
+                          double* q_solvent, double* phi_chain, double* mu_chain,
+                          double* phi_solvent, double* mu_solvent);
+/* Reverse mode of the last density() call (either form) of `sys`, for a
+ * scalar L with lam_rho_grid = dL/drho_grid and dL_dQ(N_chain) (host, may be
+ * NULL).  Out: grad_omega_grid = dL/domega_grid; with dGG (dim, dim,
+ * n_cell_param) of make_unit_cell (host; may be NULL) dL_dcell(n_cell_param);
+ * with dL_dlength (host; may be NULL) dL/d block length for every block of
+ * every chain, chain-major, at fixed block step counts.  Systems with
+ * solvent species are rejected. */
+int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const double* dL_dQ,
+                             double* grad_omega_grid, int n_cell_param, const double* dGG,
+                             double* dL_dcell, double* dL_dlength);
+/* basis_to_kgrid + ifft of nf star fields, device basis (nf, N_star) ->
+ * device grids (M, nf)                        src/scf_mod.fp.f90:182-185 */
+int pscf_basis_to_grid_dev(int64_t sys, const double* basis, int nf, double* grid);
+/* fft + kgrid_to_basis of nf device grids, times scale (density() uses 1/M)
+ *                                             src/scf_mod.fp.f90:221-225 */
+int pscf_grid_to_basis_dev(int64_t sys, const double* grid, int nf, double scale, double* basis);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,13 @@ SIGNATURES = {
     "pscf_density": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                      _c_dbl_p, _c_dbl_p],
     "pscf_scf_stress": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_density_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
+                              _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_density_adjoint_dev": [ctypes.c_int64, ctypes.c_void_p, _c_dbl_p, ctypes.c_void_p,
+                                 ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_basis_to_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+    "pscf_grid_to_basis_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                               ctypes.c_void_p],
     "pscf_divide_energy_terms": [ctypes.c_int64, _c_dbl_p],
     "pscf_system_get_grids": [ctypes.c_int64, _c_dbl_p, _c_dbl_p],
     "pscf_system_get_q": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p],
@@ -87,6 +94,9 @@ class PscfError(RuntimeError):
 
 class Lib:
     """A loaded libpscf_hip (or a library exporting the same C ABI)."""
+
+    # torch device whose memory the library's *_dev entry points address
+    tensor_device = "cuda"
 
     def __init__(self, path=None):
         path = path or DEFAULT_LIB

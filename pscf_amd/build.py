@@ -19,11 +19,14 @@ HIP_FLAGS = [
 ]
 
 
-def build_lib(verbose=False):
+def build_lib(verbose=False, defines=None, out=None):
+    """Compile the library; `defines` ({name: value}) set the build-time tuning
+    knobs of the kernels (PSCF_MAX_RADIX, PSCF_XPAD), `out` the output path."""
     srcs = [os.path.join(HERE, "csrc", f) for f in SOURCES]
-    out = os.path.join(HERE, "libpscf_hip.so")
+    out = out or os.path.join(HERE, "libpscf_hip.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + srcs + ["-o", tmp]
+    dflags = ["-D%s=%s" % (k, v) for k, v in sorted((defines or {}).items())]
+    cmd = [HIPCC] + HIP_FLAGS + dflags + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

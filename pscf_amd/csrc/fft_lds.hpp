@@ -29,9 +29,21 @@ template <int S> __device__ __forceinline__ double2 cmul_si(double2 a) {
 }
 
 // ------------------------------------------------------ compile-time factoring
+// Build-time tuning knobs (the variants an autotuning build sweeps):
+//   PSCF_MAX_RADIX  largest radix of a pass (8 or 4)
+//   PSCF_XPAD       complex words of padding per packed row of a plane; with
+//                   padding the x passes map consecutive threads to
+//                   consecutive rows, whose LDS addresses then fall in
+//                   different banks
+#ifndef PSCF_MAX_RADIX
+#define PSCF_MAX_RADIX 8
+#endif
+#ifndef PSCF_XPAD
+#define PSCF_XPAD 0
+#endif
 constexpr int fft_nfac(int n) {
   int c = 0;
-  while (n > 1 && n % 8 == 0) { n /= 8; ++c; }
+  while (PSCF_MAX_RADIX >= 8 && n > 1 && n % 8 == 0) { n /= 8; ++c; }
   while (n % 4 == 0) { n /= 4; ++c; }
   while (n % 2 == 0) { n /= 2; ++c; }
   while (n % 3 == 0) { n /= 3; ++c; }
@@ -40,7 +52,7 @@ constexpr int fft_nfac(int n) {
 }
 constexpr int fft_fac(int n, int p) {
   int c = 0;
-  while (n > 1 && n % 8 == 0) { if (c == p) return 8; n /= 8; ++c; }
+  while (PSCF_MAX_RADIX >= 8 && n > 1 && n % 8 == 0) { if (c == p) return 8; n /= 8; ++c; }
   while (n % 4 == 0) { if (c == p) return 4; n /= 4; ++c; }
   while (n % 2 == 0) { if (c == p) return 2; n /= 2; ++c; }
   while (n % 3 == 0) { if (c == p) return 3; n /= 3; ++c; }
@@ -201,9 +213,13 @@ __device__ __forceinline__ void fft_lines(double2* buf, const double2* __restric
 template <int N1, int N2> struct Plane {
   static constexpr int N1H = N1 / 2 + 1;
   static constexpr int NP = (N2 + 1) / 2;        // packed row pairs
-  static constexpr int PSIZE = N1 * NP;          // complex words in P
+  static constexpr int PLS = N1 + PSCF_XPAD;     // LDS stride of a packed row
+  static constexpr int PSIZE = PLS * NP;         // complex words in P
   static constexpr int HSIZE = N1H * N2;         // complex words in H
+  static constexpr bool XJFAST = PSCF_XPAD == 0; // thread mapping of the x passes
   static_assert(N1 % 2 == 0, "first grid dimension must be even");
+  // LDS index of element x of packed row pl, given i = pl*N1 + x
+  static __device__ __forceinline__ int pidx(int i) { return (i / N1) * PLS + i % N1; }
 };
 
 // P (x-transformed pairs) -> H (half rows)
@@ -213,8 +229,8 @@ __device__ __forceinline__ void split_pairs(const double2* P, double2* H, int ti
   constexpr int TOT = PL::NP * PL::N1H;
   for (int i = tid; i < TOT; i += NT) {
     const int pl = i / PL::N1H, k = i % PL::N1H;
-    const double2 z = P[pl * N1 + k];
-    const double2 zc = cconj(P[pl * N1 + (N1 - k) % N1]);
+    const double2 z = P[pl * PL::PLS + k];
+    const double2 zc = cconj(P[pl * PL::PLS + (N1 - k) % N1]);
     H[(2 * pl) * PL::N1H + k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
     if (2 * pl + 1 < N2) {
       const double2 d = csub(z, zc);                       // (z - zc) / (2i)
@@ -237,7 +253,7 @@ __device__ __forceinline__ void join_pairs(const double2* H, double2* P, int tid
     double2 b = (2 * pl + 1 < N2) ? H[(2 * pl + 1) * PL::N1H + k] : make_double2(0.0, 0.0);
     if (k == 0 || k == N1 / 2) { a.y = 0.0; b.y = 0.0; }
     if (hi) { a.y = -a.y; b.y = -b.y; }
-    P[pl * N1 + x] = make_double2(a.x - b.y, a.y + b.x);     // a + i b
+    P[pl * PL::PLS + x] = make_double2(a.x - b.y, a.y + b.x);     // a + i b
   }
   __syncthreads();
 }
@@ -247,7 +263,7 @@ template <int N1, int N2, int NT>
 __device__ __forceinline__ void plane_forward(double2* P, double2* H, const double2* tw1,
                                               const double2* tw2, int tid) {
   using PL = Plane<N1, N2>;
-  fft_lines<N1, -1, PL::NP, N1, 1, true, NT>(P, tw1, tid);
+  fft_lines<N1, -1, PL::NP, PL::PLS, 1, PL::XJFAST, NT>(P, tw1, tid);
   split_pairs<N1, N2, NT>(P, H, tid);
   fft_lines<N2, -1, PL::N1H, 1, PL::N1H, false, NT>(H, tw2, tid);
 }
@@ -259,7 +275,7 @@ __device__ __forceinline__ void plane_inverse(double2* H, double2* P, const doub
   using PL = Plane<N1, N2>;
   fft_lines<N2, +1, PL::N1H, 1, PL::N1H, false, NT>(H, tw2, tid);
   join_pairs<N1, N2, NT>(H, P, tid);
-  fft_lines<N1, +1, PL::NP, N1, 1, true, NT>(P, tw1, tid);
+  fft_lines<N1, +1, PL::NP, PL::PLS, 1, PL::XJFAST, NT>(P, tw1, tid);
 }
 
 }  // namespace pscf

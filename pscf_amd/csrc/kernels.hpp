@@ -16,8 +16,25 @@
 
 namespace pscf {
 
-constexpr int NT = 256;        // threads per workgroup for every kernel below
 constexpr int MAXBLK = 16;     // blocks per chain
+
+// Workgroup size of a kernel: enough threads for every butterfly of the first
+// (radix-8) transform pass to run at once -- e.g. 33 half-spectrum lines of
+// 64 points have 264 radix-8 butterflies, so 320 threads rather than 256,
+// which would leave one wave doing a second butterfly alone in every pass.
+constexpr int round64(int n) { return ((n + 63) / 64) * 64; }
+constexpr int clampnt(int n) { return n < 64 ? 64 : (n > 1024 ? 1024 : n); }
+constexpr int plane_nt(int n1, int n2) {
+  const int x = ((n2 + 1) / 2) * (n1 / fft_fac(n1, 0));
+  const int y = n2 > 1 ? (n1 / 2 + 1) * (n2 / fft_fac(n2, 0)) : 0;
+  return clampnt(round64(x > y ? x : y));
+}
+constexpr int zcol_nt(int w, int n3) { return n3 > 1 ? clampnt(round64(w * (n3 / fft_fac(n3, 0)))) : 64; }
+constexpr int c2c_nt(int n1, int n2) {
+  const int x = n2 * (n1 / fft_fac(n1, 0));
+  const int y = n2 > 1 ? n1 * (n2 / fft_fac(n2, 0)) : 0;
+  return clampnt(round64(x > y ? x : y));
+}
 
 // One contour sweep (a chain's q or q-dagger) advanced by the step kernels.
 // Step j maps contour index in(j) -> out(j): forward in = j, backward
@@ -29,6 +46,10 @@ struct StepTask {
   int dir;                      // +1 forward sweep, -1 backward sweep
   int init_one;                 // 1: the first point is the constant 1 (and is written)
   int skip;                     // 1: inactive (converged instance of a batch): every kernel returns
+  int rev;                      // 1: step j is the step j' = ns-1-j (adjoint recomputation)
+  int reload;                   // 1: every step reads its input from the slab (no carry)
+  int add_src;                  // 1: out = T in + src[out] (adjoint sweep with sources)
+  int nostore;                  // 1: step outputs are not written to the slab
   int nblk;
   int bnd[MAXBLK + 1];
   const double* e1[MAXBLK];     // exp(-ds/2 w)   (M)
@@ -39,107 +60,201 @@ struct StepTask {
   double2* B;                   // first half-step spectrum  (Mk)
   double2* C;                   // second half-step spectrum (Mk)
   double* q1;                   // full-step result          (M)
+  const double* src;            // source slab (add_src; its first point is also the start value)
+  // adjoint support: pre-factor fields of the step kept for the gradient,
+  // U = C1[e1 q], V = C2[e2 q], W = C2[e1 V] (C: normalised convolutions)
+  double* keepU;
+  double* keepV;
+  double* keepW;
+  double2* keepK[3];            // spectra FFT[e1 q], FFT[e2 q], FFT[e1 V] (Mk, before the k factor)
+  double ds[MAXBLK];            // block step sizes (adjoint gradient)
+  double bsq[MAXBLK];           // squared statistical segment length of the block's monomer
+  double* grad[MAXBLK];         // omega gradient of the block (M)
+  double* kgrad[MAXBLK];        // |k|^2 gradient of the block on the half grid (Mk)
 };
 
-__device__ __forceinline__ int task_in(const StepTask& t, int j) { return t.dir > 0 ? j : t.ns - j; }
-__device__ __forceinline__ int task_out(const StepTask& t, int j) { return t.dir > 0 ? j + 1 : t.ns - j - 1; }
+__device__ __forceinline__ int task_j(const StepTask& t, int j) { return t.rev ? t.ns - 1 - j : j; }
+__device__ __forceinline__ int task_in(const StepTask& t, int j) {
+  j = task_j(t, j);
+  return t.dir > 0 ? j : t.ns - j;
+}
+__device__ __forceinline__ int task_out(const StepTask& t, int j) {
+  j = task_j(t, j);
+  return t.dir > 0 ? j + 1 : t.ns - j - 1;
+}
 __device__ __forceinline__ int task_block(const StepTask& t, int j) {
+  j = task_j(t, j);
   const int lo = t.dir > 0 ? j : t.ns - j - 1;
   int b = 0;
   while (b + 1 < t.nblk && lo >= t.bnd[b + 1]) ++b;
   return b;
 }
 
+// Copy n complex words global -> LDS (no barrier).
+template <int N, int NT>
+__device__ __forceinline__ void stage(double2* dst, const double2* __restrict__ src, int tid) {
+#pragma unroll
+  for (int p = 0; p < (N + NT - 1) / NT; ++p) {
+    const int i = tid + p * NT;
+    if (i < N) dst[i] = src[i];
+  }
+}
+
 // ----------------------------------------------------------------------------
-// Step kernel A (one workgroup per z plane and sweep): finishes step j-1 with
-// the second half step's x,y inverse transform, exp(-ds w/4)/M and the
-// Richardson combination q = (4 q2 - q1)/3 (stored to the slab), then starts
-// step j: exp(-ds w/2) q and exp(-ds w/4) q, each x,y forward transformed.
-// Reference: src/step_mod.f90:156-174.
+// Step kernel A (two workgroups per z plane and sweep).  Both finish step j-1:
+// x,y inverse transform of the second half step's spectrum C, times
+// exp(-ds w/4)/M, Richardson q = (4 q2 - q1)/3 (job 0 stores it to the slab).
+// Then step j starts: job 0 transforms exp(-ds w/2) q into A, job 1
+// exp(-ds w/4) q into B.  Reference: src/step_mod.f90:156-174.
+// Every global input of the thread is requested at kernel entry, and the
+// twiddles are staged in LDS, so the transform passes wait on LDS only.
 // ----------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ void __launch_bounds__(NT) k_step_a(const StepTask* __restrict__ tasks, int j, int N3,
-                                               const double2* __restrict__ tw1,
-                                               const double2* __restrict__ tw2) {
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __restrict__ tasks, int j, int N3,
+                                               const double2* __restrict__ tw1g,
+                                               const double2* __restrict__ tw2g) {
+  constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
-  constexpr int PER = (PL::NP * N1 + NT - 1) / NT;
+  constexpr int NPX = PL::NP * N1;
+  constexpr int PER = (NPX + NT - 1) / NT;
   __shared__ double2 P[PL::PSIZE];
   __shared__ double2 H[PL::HSIZE];
-  const int tid = threadIdx.x, z = blockIdx.x;
+  __shared__ double2 TW1[N1], TW2[N2];
+  const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
-  if (t.skip || j > t.ns) return;
+  if (t.skip || j > t.ns || (job == 1 && j == t.ns)) return;
   const long M = (long)N1 * N2 * N3;
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
   const long hbase = (long)z * PL::HSIZE;
-  double qa[PER], qb[PER];
-  if (j == 0) {
-    double* qin = t.slab + (long)task_in(t, 0) * M + pbase;
+  const bool start = j < t.ns;
+  const double* ef = start ? (job == 0 ? t.e1[task_block(t, j)] : t.e2[task_block(t, j)]) + pbase
+                           : nullptr;
+  double qa[PER], qb[PER], fa[PER], fb[PER], e2a[PER], e2b[PER], q1a[PER], q1b[PER];
+  // requests first: factors of this step, and for j > 0 the previous step's inputs
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    fa[p] = fb[p] = e2a[p] = e2b[p] = q1a[p] = q1b[p] = 0.0;
+    if (i < NPX) {
+      const int o0 = 2 * (i / N1) * N1 + i % N1;
+      const bool two = 2 * (i / N1) + 1 < N2;
+      if (start) {
+        fa[p] = ef[o0];
+        if (two) fb[p] = ef[o0 + N1];
+      }
+      if (j > 0) {
+        const double* e2 = t.e2[task_block(t, j - 1)] + pbase;
+        const double* q1 = t.q1 + pbase;
+        e2a[p] = e2[o0];
+        q1a[p] = q1[o0];
+        if (two) {
+          e2b[p] = e2[o0 + N1];
+          q1b[p] = q1[o0 + N1];
+        }
+      }
+    }
+  }
+  stage<N1, NT>(TW1, tw1g, tid);
+  stage<N2, NT>(TW2, tw2g, tid);
+  if (t.reload && start) {
+    const double* qin = t.slab + (long)task_in(t, j) * M + pbase;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int i = tid + p * NT;
-      if (i < PL::NP * N1) {
-        const int x = i % N1, y0 = 2 * (i / N1);
+      qa[p] = qb[p] = 0.0;
+      if (i < NPX) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        qa[p] = qin[o0];
+        if (2 * (i / N1) + 1 < N2) qb[p] = qin[o0 + N1];
+      }
+    }
+  }
+  if (j == 0) {
+    double* qin = t.slab + (long)task_in(t, 0) * M + pbase;
+    const double* s0 = t.add_src ? t.src + (long)task_in(t, 0) * M + pbase : nullptr;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = tid + p * NT;
+      if (i < NPX) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        const bool two = 2 * (i / N1) + 1 < N2;
         if (t.init_one) {
           qa[p] = 1.0;
           qb[p] = 1.0;
-          qin[y0 * N1 + x] = 1.0;
-          if (y0 + 1 < N2) qin[(y0 + 1) * N1 + x] = 1.0;
+          if (job == 0) {
+            qin[o0] = 1.0;
+            if (two) qin[o0 + N1] = 1.0;
+          }
+        } else if (s0) {
+          qa[p] = s0[o0];
+          qb[p] = two ? s0[o0 + N1] : 0.0;
+          if (job == 0) {
+            qin[o0] = qa[p];
+            if (two) qin[o0 + N1] = qb[p];
+          }
         } else {
-          qa[p] = qin[y0 * N1 + x];
-          qb[p] = (y0 + 1 < N2) ? qin[(y0 + 1) * N1 + x] : 0.0;
+          qa[p] = qin[o0];
+          qb[p] = two ? qin[o0 + N1] : 0.0;
         }
       }
     }
   } else {
-    const int bp = task_block(t, j - 1);
-    const double* e2 = t.e2[bp] + pbase;
-    const double* q1 = t.q1 + pbase;
-    double* qo = t.slab + (long)task_out(t, j - 1) * M + pbase;
-    for (int i = tid; i < PL::HSIZE; i += NT) H[i] = t.C[hbase + i];
+    stage<PL::HSIZE, NT>(H, t.C + hbase, tid);
     __syncthreads();
-    plane_inverse<N1, N2, NT>(H, P, tw1, tw2, tid);
+    plane_inverse<N1, N2, NT>(H, P, TW1, TW2, tid);
+    double* qo = t.slab + (long)task_out(t, j - 1) * M + pbase;
+    const double* so = t.add_src ? t.src + (long)task_out(t, j - 1) * M + pbase : nullptr;
+    double* kw = t.keepW ? t.keepW + pbase : nullptr;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int i = tid + p * NT;
-      if (i < PL::NP * N1) {
-        const int x = i % N1, y0 = 2 * (i / N1);
-        const double2 r = P[i];
-        const int o0 = y0 * N1 + x;
-        const double q2a = e2[o0] * r.x / rM;
-        qa[p] = (4.0 * q2a - q1[o0]) / 3.0;
-        qo[o0] = qa[p];
-        if (y0 + 1 < N2) {
-          const int o1 = o0 + N1;
-          const double q2b = e2[o1] * r.y / rM;
-          qb[p] = (4.0 * q2b - q1[o1]) / 3.0;
-          qo[o1] = qb[p];
-        } else {
-          qb[p] = 0.0;
+      if (i < NPX) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        const bool two = 2 * (i / N1) + 1 < N2;
+        const double2 r = P[PL::pidx(i)];
+        const double q2a = e2a[p] * r.x / rM;
+        double va = (4.0 * q2a - q1a[p]) / 3.0;
+        double vb = 0.0;
+        if (two) {
+          const double q2b = e2b[p] * r.y / rM;
+          vb = (4.0 * q2b - q1b[p]) / 3.0;
+        }
+        if (so) {
+          va += so[o0];
+          if (two) vb += so[o0 + N1];
+        }
+        if (!t.reload) {
+          qa[p] = va;
+          qb[p] = vb;
+        }
+        if (job == 0) {
+          if (!t.nostore) {
+            qo[o0] = va;
+            if (two) qo[o0 + N1] = vb;
+          }
+          if (kw) {
+            kw[o0] = r.x / rM;
+            if (two) kw[o0 + N1] = r.y / rM;
+          }
         }
       }
     }
     __syncthreads();
   }
-  if (j == t.ns) return;
-  const int b = task_block(t, j);
+  if (!start) return;
 #pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const double* e = (half == 0 ? t.e1[b] : t.e2[b]) + pbase;
-    double2* out = (half == 0 ? t.A : t.B) + hbase;
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < NPX) P[PL::pidx(i)] = make_double2(fa[p] * qa[p], fb[p] * qb[p]);
+  }
+  __syncthreads();
+  plane_forward<N1, N2, NT>(P, H, TW1, TW2, tid);
+  double2* out = (job == 0 ? t.A : t.B) + hbase;
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int i = tid + p * NT;
-      if (i < PL::NP * N1) {
-        const int x = i % N1, y0 = 2 * (i / N1);
-        const int o0 = y0 * N1 + x;
-        P[i] = make_double2(e[o0] * qa[p], (y0 + 1 < N2) ? e[o0 + N1] * qb[p] : 0.0);
-      }
-    }
-    __syncthreads();
-    plane_forward<N1, N2, NT>(P, H, tw1, tw2, tid);
-    for (int i = tid; i < PL::HSIZE; i += NT) out[i] = H[i];
-    __syncthreads();
+  for (int p = 0; p < (PL::HSIZE + NT - 1) / NT; ++p) {
+    const int i = tid + p * NT;
+    if (i < PL::HSIZE) out[i] = H[i];
   }
 }
 
@@ -150,12 +265,16 @@ __global__ void __launch_bounds__(NT) k_step_a(const StepTask* __restrict__ task
 // transform into C.  Reference: src/step_mod.f90:159-167.
 // ----------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ void __launch_bounds__(NT) k_step_c(const StepTask* __restrict__ tasks, int j, int N3,
-                                               const double2* __restrict__ tw1,
-                                               const double2* __restrict__ tw2) {
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __restrict__ tasks, int j, int N3,
+                                               const double2* __restrict__ tw1g,
+                                               const double2* __restrict__ tw2g) {
+  constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
+  constexpr int NPX = PL::NP * N1;
+  constexpr int PER = (NPX + NT - 1) / NT;
   __shared__ double2 P[PL::PSIZE];
   __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 TW1[N1], TW2[N2];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
   if (t.skip || j >= t.ns) return;
@@ -163,33 +282,65 @@ __global__ void __launch_bounds__(NT) k_step_c(const StepTask* __restrict__ task
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
   const long hbase = (long)z * PL::HSIZE;
-  const int b = task_block(t, j);
-  const double* e1 = t.e1[b] + pbase;
-  const double2* src = (job == 0 ? t.A : t.B) + hbase;
-  for (int i = tid; i < PL::HSIZE; i += NT) H[i] = src[i];
+  const double* e1 = t.e1[task_block(t, j)] + pbase;
+  double fa[PER], fb[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    fa[p] = fb[p] = 0.0;
+    if (i < NPX) {
+      const int o0 = 2 * (i / N1) * N1 + i % N1;
+      fa[p] = e1[o0];
+      if (2 * (i / N1) + 1 < N2) fb[p] = e1[o0 + N1];
+    }
+  }
+  stage<PL::HSIZE, NT>(H, (job == 0 ? t.A : t.B) + hbase, tid);
+  stage<N1, NT>(TW1, tw1g, tid);
+  stage<N2, NT>(TW2, tw2g, tid);
   __syncthreads();
-  plane_inverse<N1, N2, NT>(H, P, tw1, tw2, tid);
+  plane_inverse<N1, N2, NT>(H, P, TW1, TW2, tid);
+  double* keep = (job == 0 ? t.keepU : t.keepV);
+  if (keep) keep += pbase;
   if (job == 0) {
     double* q1 = t.q1 + pbase;
-    for (int i = tid; i < PL::NP * N1; i += NT) {
-      const int x = i % N1, y0 = 2 * (i / N1);
-      const int o0 = y0 * N1 + x;
-      const double2 r = P[i];
-      q1[o0] = e1[o0] * r.x / rM;
-      if (y0 + 1 < N2) q1[o0 + N1] = e1[o0 + N1] * r.y / rM;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = tid + p * NT;
+      if (i < NPX) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        const bool two = 2 * (i / N1) + 1 < N2;
+        const double2 r = P[PL::pidx(i)];
+        q1[o0] = fa[p] * r.x / rM;
+        if (two) q1[o0 + N1] = fb[p] * r.y / rM;
+        if (keep) {
+          keep[o0] = r.x / rM;
+          if (two) keep[o0 + N1] = r.y / rM;
+        }
+      }
     }
     return;
   }
-  for (int i = tid; i < PL::NP * N1; i += NT) {
-    const int x = i % N1, y0 = 2 * (i / N1);
-    const int o0 = y0 * N1 + x;
-    const double2 r = P[i];
-    P[i] = make_double2(e1[o0] * r.x / rM, (y0 + 1 < N2) ? e1[o0 + N1] * r.y / rM : 0.0);
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < NPX) {
+      const double2 r = P[PL::pidx(i)];
+      if (keep) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        keep[o0] = r.x / rM;
+        if (2 * (i / N1) + 1 < N2) keep[o0 + N1] = r.y / rM;
+      }
+      P[PL::pidx(i)] = make_double2(fa[p] * r.x / rM, fb[p] * r.y / rM);
+    }
   }
   __syncthreads();
-  plane_forward<N1, N2, NT>(P, H, tw1, tw2, tid);
+  plane_forward<N1, N2, NT>(P, H, TW1, TW2, tid);
   double2* out = t.C + hbase;
-  for (int i = tid; i < PL::HSIZE; i += NT) out[i] = H[i];
+#pragma unroll
+  for (int p = 0; p < (PL::HSIZE + NT - 1) / NT; ++p) {
+    const int i = tid + p * NT;
+    if (i < PL::HSIZE) out[i] = H[i];
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -198,10 +349,14 @@ __global__ void __launch_bounds__(NT) k_step_c(const StepTask* __restrict__ task
 // Jobs: kind 0 -> (A, k1) and (B, k2); kind 1 -> (C, k2).
 // ----------------------------------------------------------------------------
 template <int N1, int N3>
-__global__ void __launch_bounds__(NT) k_step_z(const StepTask* __restrict__ tasks, int j, int N2,
-                                               int kind, const double2* __restrict__ tw3) {
+__global__ void __launch_bounds__(zcol_nt(N1 / 2 + 1, N3)) k_step_z(const StepTask* __restrict__ tasks, int j, int N2,
+                                               int kind, const double2* __restrict__ tw3g) {
+  constexpr int NT = zcol_nt(N1 / 2 + 1, N3);
   constexpr int N1H = N1 / 2 + 1;
-  __shared__ double2 T[N1H * N3];
+  constexpr int TOT = N1H * N3;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  __shared__ double2 T[TOT];
+  __shared__ double2 TW3[N3];
   const int tid = threadIdx.x, ky = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
   if (t.skip || j >= t.ns) return;
@@ -216,21 +371,96 @@ __global__ void __launch_bounds__(NT) k_step_z(const StepTask* __restrict__ task
     kf = t.k2[b];
   }
   const long rowstride = (long)N1H * N2;
-  for (int i = tid; i < N1H * N3; i += NT) {
-    const int kz = i / N1H, kx = i % N1H;
-    T[i] = arr[kz * rowstride + (long)ky * N1H + kx];
+  double f[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) {
+      const long g = (i / N1H) * rowstride + (long)ky * N1H + i % N1H;
+      T[i] = arr[g];
+      f[p] = kf[g];
+    }
+  }
+  stage<N3, NT>(TW3, tw3g, tid);
+  __syncthreads();
+  fft_lines<N3, -1, N1H, 1, N1H, false, NT>(T, TW3, tid);
+  double2* keep = t.keepK[kind == 0 ? job : 2];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) {
+      if (keep) keep[(i / N1H) * rowstride + (long)ky * N1H + i % N1H] = T[i];
+      T[i] = cscale(T[i], f[p]);
+    }
   }
   __syncthreads();
-  fft_lines<N3, -1, N1H, 1, N1H, false, NT>(T, tw3, tid);
-  for (int i = tid; i < N1H * N3; i += NT) {
-    const int kz = i / N1H, kx = i % N1H;
-    T[i] = cscale(T[i], kf[kz * rowstride + (long)ky * N1H + kx]);
+  fft_lines<N3, +1, N1H, 1, N1H, false, NT>(T, TW3, tid);
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) arr[(i / N1H) * rowstride + (long)ky * N1H + i % N1H] = T[i];
   }
-  __syncthreads();
-  fft_lines<N3, +1, N1H, 1, N1H, false, NT>(T, tw3, tid);
-  for (int i = tid; i < N1H * N3; i += NT) {
-    const int kz = i / N1H, kx = i % N1H;
-    arr[kz * rowstride + (long)ky * N1H + kx] = T[i];
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint gradient of one step for each (adjoint task, recomputed task) pair.
+// With a the adjoint entering the step and q the field the step is applied
+// to (both step inputs, read from their slabs), and U, V, W the kept
+// pre-factor fields of the step on q (suffix q) and on a (suffix a):
+//   d<a, T q>/d omega = -(ds/3) [ e2 (a Wq + q Wa) + 2 e1 Va Vq - (e1/2)(a Uq + q Ua) ]
+// for T = (4 e2 C2 e1 C2 e2 - e1 C1 e1)/3 (src/step_mod.f90:156-174),
+// accumulated into the omega gradient of the step's monomer.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ tasks,
+                                                  const int2* __restrict__ pairs, int j, long m) {
+  const StepTask& A = tasks[pairs[blockIdx.y].x];
+  const StepTask& Q = tasks[pairs[blockIdx.y].y];
+  if (A.skip || j < 0 || j >= A.ns) return;
+  const int b = task_block(A, j);
+  const double* a = A.slab + (long)task_in(A, j) * m;
+  const double* q = Q.slab + (long)task_in(Q, j) * m;
+  const double* e1 = A.e1[b];
+  const double* e2 = A.e2[b];
+  const double c = A.ds[b] / 3.0;
+  double* grad = A.grad[b];
+  for (long r = blockIdx.x * 256L + threadIdx.x; r < m; r += (long)gridDim.x * 256L) {
+    const double av = a[r], qv = q[r];
+    const double g = e2[r] * (av * Q.keepW[r] + qv * A.keepW[r]) +
+                     2.0 * e1[r] * A.keepV[r] * Q.keepV[r] -
+                     0.5 * e1[r] * (av * Q.keepU[r] + qv * A.keepU[r]);
+    grad[r] -= c * g;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Reciprocal-space part of the same step derivative.  With the unnormalised
+// 3D spectra Aq = FFT[e1 q], Bq = FFT[e2 q], Cq = FFT[e1 Vq] (and likewise for
+// a), and k1 = exp(-ds b^2 k^2/6), k2 = exp(-ds b^2 k^2/12):
+//   d<a,Tq>/dk1(k) = -(1/3M) Re(Aq Aa*),  d<a,Tq>/dk2(k) = (4/3M) Re(Cq Ba* + Bq Ca*)
+// so d<a,Tq>/d|k|^2 = (ds b^2/M) [ k1 Re(Aq Aa*)/18 - k2 Re(Cq Ba* + Bq Ca*)/9 ].
+// A half-grid point with 0 < kx < N1/2 stands for k and -k and counts twice.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adj_kgrad(const StepTask* __restrict__ tasks,
+                                                   const int2* __restrict__ pairs, int j, int n1,
+                                                   long mk, double rM) {
+  const StepTask& A = tasks[pairs[blockIdx.y].x];
+  const StepTask& Q = tasks[pairs[blockIdx.y].y];
+  if (A.skip || j < 0 || j >= A.ns) return;
+  const int b = task_block(A, j);
+  const double* k1 = A.k1[b];
+  const double* k2 = A.k2[b];
+  const double c = A.ds[b] * A.bsq[b] / rM;
+  const int n1h = n1 / 2 + 1;
+  double* kg = A.kgrad[b];
+  for (long g = blockIdx.x * 256L + threadIdx.x; g < mk; g += (long)gridDim.x * 256L) {
+    const int kx = (int)(g % n1h);
+    const double mult = (kx == 0 || 2 * kx == n1) ? 1.0 : 2.0;
+    const double2 aq = Q.keepK[0][g], aa = A.keepK[0][g];
+    const double2 bq = Q.keepK[1][g], ba = A.keepK[1][g];
+    const double2 cq = Q.keepK[2][g], ca = A.keepK[2][g];
+    const double r1 = aq.x * aa.x + aq.y * aa.y;
+    const double r2 = cq.x * ba.x + cq.y * ba.y + bq.x * ca.x + bq.y * ca.y;
+    kg[g] += mult * c * (k1[g] * r1 / 18.0 - k2[g] * r2 / 9.0);
   }
 }
 
@@ -240,18 +470,22 @@ __global__ void __launch_bounds__(NT) k_step_z(const StepTask* __restrict__ task
 
 // real planes -> x,y forward transformed half planes; in[b*istride + ...]
 template <int N1, int N2>
-__global__ void __launch_bounds__(NT) k_plane_r2c(const double* __restrict__ in, long istride,
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_r2c(const double* __restrict__ in, long istride,
                                                   double2* __restrict__ out, long ostride,
-                                                  const double2* __restrict__ tw1,
-                                                  const double2* __restrict__ tw2) {
+                                                  const double2* __restrict__ tw1g,
+                                                  const double2* __restrict__ tw2g) {
+  constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
   __shared__ double2 P[PL::PSIZE];
   __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 tw1[N1], tw2[N2];
   const int tid = threadIdx.x, z = blockIdx.x;
+  stage<N1, NT>(tw1, tw1g, tid);
+  stage<N2, NT>(tw2, tw2g, tid);
   const double* src = in + blockIdx.y * istride + (long)z * N1 * N2;
   for (int i = tid; i < PL::NP * N1; i += NT) {
     const int x = i % N1, y0 = 2 * (i / N1);
-    P[i] = make_double2(src[y0 * N1 + x], (y0 + 1 < N2) ? src[(y0 + 1) * N1 + x] : 0.0);
+    P[PL::pidx(i)] = make_double2(src[y0 * N1 + x], (y0 + 1 < N2) ? src[(y0 + 1) * N1 + x] : 0.0);
   }
   __syncthreads();
   plane_forward<N1, N2, NT>(P, H, tw1, tw2, tid);
@@ -261,14 +495,18 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const double* __restrict__ in,
 
 // half planes -> x,y inverse transformed real planes, times `scale`
 template <int N1, int N2>
-__global__ void __launch_bounds__(NT) k_plane_c2r(const double2* __restrict__ in, long istride,
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_c2r(const double2* __restrict__ in, long istride,
                                                   double* __restrict__ out, long ostride,
-                                                  double scale, const double2* __restrict__ tw1,
-                                                  const double2* __restrict__ tw2) {
+                                                  double scale, const double2* __restrict__ tw1g,
+                                                  const double2* __restrict__ tw2g) {
+  constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
   __shared__ double2 P[PL::PSIZE];
   __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 tw1[N1], tw2[N2];
   const int tid = threadIdx.x, z = blockIdx.x;
+  stage<N1, NT>(tw1, tw1g, tid);
+  stage<N2, NT>(tw2, tw2g, tid);
   const double2* src = in + blockIdx.y * istride + (long)z * PL::HSIZE;
   for (int i = tid; i < PL::HSIZE; i += NT) H[i] = src[i];
   __syncthreads();
@@ -276,7 +514,7 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const double2* __restrict__ in
   double* dst = out + blockIdx.y * ostride + (long)z * N1 * N2;
   for (int i = tid; i < PL::NP * N1; i += NT) {
     const int x = i % N1, y0 = 2 * (i / N1);
-    const double2 r = P[i];
+    const double2 r = P[PL::pidx(i)];
     dst[y0 * N1 + x] = r.x * scale;
     if (y0 + 1 < N2) dst[(y0 + 1) * N1 + x] = r.y * scale;
   }
@@ -285,10 +523,13 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const double2* __restrict__ in
 // in-place z transform of W-wide rows (W = N1/2+1 for half spectra, N1 for
 // full complex grids), batched over blockIdx.y arrays
 template <int W, int N3, int SIGN>
-__global__ void __launch_bounds__(NT) k_zfft(double2* __restrict__ data, long stride, int N2,
-                                             const double2* __restrict__ tw3) {
+__global__ void __launch_bounds__(zcol_nt(W, N3)) k_zfft(double2* __restrict__ data, long stride, int N2,
+                                             const double2* __restrict__ tw3g) {
+  constexpr int NT = zcol_nt(W, N3);
   __shared__ double2 T[W * N3];
+  __shared__ double2 tw3[N3];
   const int tid = threadIdx.x, ky = blockIdx.x;
+  stage<N3, NT>(tw3, tw3g, tid);
   double2* arr = data + blockIdx.y * stride;
   const long rowstride = (long)W * N2;
   for (int i = tid; i < W * N3; i += NT) T[i] = arr[(i / W) * rowstride + (long)ky * W + i % W];
@@ -299,11 +540,15 @@ __global__ void __launch_bounds__(NT) k_zfft(double2* __restrict__ data, long st
 
 // full complex planes, in place, x then y (used by the c2c transform)
 template <int N1, int N2, int SIGN>
-__global__ void __launch_bounds__(NT) k_plane_c2c(double2* __restrict__ data, long stride,
-                                                  const double2* __restrict__ tw1,
-                                                  const double2* __restrict__ tw2) {
+__global__ void __launch_bounds__(c2c_nt(N1, N2)) k_plane_c2c(double2* __restrict__ data, long stride,
+                                                  const double2* __restrict__ tw1g,
+                                                  const double2* __restrict__ tw2g) {
+  constexpr int NT = c2c_nt(N1, N2);
   __shared__ double2 T[N1 * N2];
+  __shared__ double2 tw1[N1], tw2[N2];
   const int tid = threadIdx.x, z = blockIdx.x;
+  stage<N1, NT>(tw1, tw1g, tid);
+  stage<N2, NT>(tw2, tw2g, tid);
   double2* arr = data + blockIdx.y * stride + (long)z * N1 * N2;
   for (int i = tid; i < N1 * N2; i += NT) T[i] = arr[i];
   __syncthreads();

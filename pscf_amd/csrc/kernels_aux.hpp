@@ -23,36 +23,44 @@ struct GeomArgs {
   int gmin[3], gext[3];  // dense wave table box: G in [gmin, gmin + gext)
 };
 
+// minimum image of half-grid point idx: best (integer wavevector) and |G|^2
+__device__ __forceinline__ double g_to_bz_point(const GeomArgs& g, long idx, int best[3]) {
+#pragma clang fp contract(off)
+  const int n1h = g.n1 / 2 + 1;
+  const int G[3] = {(int)(idx % n1h), (int)((idx / n1h) % g.n2), (int)(idx / ((long)n1h * g.n2))};
+  const int ng[3] = {g.n1, g.n2, g.n3};
+  best[0] = best[1] = best[2] = 0;
+  double bsq = 1.0e10;
+  const int nt = g.dim == 1 ? 3 : (g.dim == 2 ? 9 : 27);
+  for (int t = 0; t < nt; ++t) {
+    int sh[3] = {0, 0, 0};
+    int r = t;
+    for (int d = g.dim - 1; d >= 0; --d) { sh[d] = 1 - r % 3; r /= 3; }
+    int Gt[3] = {0, 0, 0};
+    for (int d = 0; d < g.dim; ++d) Gt[d] = G[d] + sh[d] * ng[d];
+    double gsq = 0.0;
+    double gv[3];
+    for (int jj = 0; jj < g.dim; ++jj) {
+      double s = 0.0;
+      for (int ii = 0; ii < g.dim; ++ii) s = s + (double)Gt[ii] * g.gb[ii * 3 + jj];
+      gv[jj] = s;
+    }
+    for (int jj = 0; jj < g.dim; ++jj) gsq = gsq + gv[jj] * gv[jj];
+    if (gsq < bsq - 1.0e-8) {
+      bsq = gsq;
+      for (int d = 0; d < g.dim; ++d) best[d] = Gt[d];
+    }
+  }
+  return bsq;
+}
+
 __global__ void __launch_bounds__(NTA) k_g_to_bz(GeomArgs g, const int* __restrict__ wtab,
                                                  double* __restrict__ ksq, int* __restrict__ kwave) {
-#pragma clang fp contract(off)
   const int n1h = g.n1 / 2 + 1;
   const long mk = (long)n1h * g.n2 * g.n3;
   for (long idx = blockIdx.x * (long)NTA + threadIdx.x; idx < mk; idx += (long)gridDim.x * NTA) {
-    const int G[3] = {(int)(idx % n1h), (int)((idx / n1h) % g.n2), (int)(idx / ((long)n1h * g.n2))};
-    const int ng[3] = {g.n1, g.n2, g.n3};
-    int best[3] = {0, 0, 0};
-    double bsq = 1.0e10;
-    const int nt = g.dim == 1 ? 3 : (g.dim == 2 ? 9 : 27);
-    for (int t = 0; t < nt; ++t) {
-      int sh[3] = {0, 0, 0};
-      int r = t;
-      for (int d = g.dim - 1; d >= 0; --d) { sh[d] = 1 - r % 3; r /= 3; }
-      int Gt[3] = {0, 0, 0};
-      for (int d = 0; d < g.dim; ++d) Gt[d] = G[d] + sh[d] * ng[d];
-      double gsq = 0.0;
-      double gv[3];
-      for (int jj = 0; jj < g.dim; ++jj) {
-        double s = 0.0;
-        for (int ii = 0; ii < g.dim; ++ii) s = s + (double)Gt[ii] * g.gb[ii * 3 + jj];
-        gv[jj] = s;
-      }
-      for (int jj = 0; jj < g.dim; ++jj) gsq = gsq + gv[jj] * gv[jj];
-      if (gsq < bsq - 1.0e-8) {
-        bsq = gsq;
-        for (int d = 0; d < g.dim; ++d) best[d] = Gt[d];
-      }
-    }
+    int best[3];
+    const double bsq = g_to_bz_point(g, idx, best);
     ksq[idx] = bsq;
     int w = 0;
     bool inside = true;
@@ -292,6 +300,115 @@ __global__ void __launch_bounds__(NTA) k_partial_qlogq(const double* __restrict_
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint of density() with respect to the omega grid fields.
+//
+// Sources (derivative of L(phi, ln Q) with respect to each stored q):
+//   g_f(s) = c sum_{b containing s} (ds_b/3) w_b(s) lam_{mon b} q_r(s)  [+ Gam/(Q M) at s = ns]
+//   g_r(s) = c sum_{b containing s} (ds_b/3) w_b(s) lam_{mon b} q_f(s)
+// with c = phi/(N Q), Simpson weights w = 1 at block ends, 4 at odd and 2 at
+// even interior offsets (src/scf_mod.fp.f90:362-424), junction points in both
+// blocks; written over the whole adjoint slabs af, ar ((ns+1) M each).
+// ----------------------------------------------------------------------------
+struct AdjSrcArgs {
+  int nblk, ns;
+  int bnd[17];
+  double ds[16];
+  int monomer[16];
+  double coef;          // phi / (N Q)
+  double endterm;       // Gam / (Q M), added to g_f(ns)
+};
+
+__global__ void __launch_bounds__(NTA) k_adj_sources(const double* __restrict__ qf,
+                                                     const double* __restrict__ qr, long m,
+                                                     AdjSrcArgs a, const double* __restrict__ lam,
+                                                     double* __restrict__ af,
+                                                     double* __restrict__ ar) {
+  const long n = (long)(a.ns + 1) * m;
+  for (long gi = blockIdx.x * (long)NTA + threadIdx.x; gi < n; gi += (long)gridDim.x * NTA) {
+    const int s = (int)(gi / m);
+    const long r = gi % m;
+    double wsum = 0.0;
+    for (int b = 0; b < a.nblk; ++b) {
+      const int lo = a.bnd[b], hi = a.bnd[b + 1];
+      if (s < lo || s > hi) continue;
+      double w = 1.0;
+      if (s != lo && s != hi) w = ((s - lo) % 2 == 1) ? 4.0 : 2.0;
+      wsum += a.ds[b] / 3.0 * w * lam[(long)a.monomer[b] * m + r];
+    }
+    const double c = a.coef * wsum;
+    af[gi] = c * qr[gi] + (s == a.ns ? a.endterm : 0.0);
+    ar[gi] = c * qf[gi];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint support.
+// k_simpson_lam: per block b of a chain (blockIdx.y), workgroup partials of
+//   T_b = ds_b/3 sum_r lam_mon(b)(r) [qf qr(ibgn) + qf qr(iend) + 4 sum_odd + 2 sum_even]
+// i.e. the block's Simpson density weighted by lam (part[b*gridDim.x + x]).
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTA) k_simpson_lam(const double* __restrict__ qf,
+                                                     const double* __restrict__ qr, long m,
+                                                     SimpsonArgs a, const double* __restrict__ lam,
+                                                     double* __restrict__ part) {
+  __shared__ double red[NTA];
+  const int b = blockIdx.y;
+  const int ibgn = a.bnd[b], iend = a.bnd[b + 1];
+  const double* lm = lam + (long)a.monomer[b] * m;
+  double tot = 0.0;
+  for (long r = blockIdx.x * (long)NTA + threadIdx.x; r < m; r += (long)gridDim.x * NTA) {
+    double acc = qf[(long)ibgn * m + r] * qr[(long)ibgn * m + r];
+    acc += qf[(long)iend * m + r] * qr[(long)iend * m + r];
+    for (int jj = ibgn + 1; jj <= iend - 1; jj += 2) acc += 4.0 * qf[(long)jj * m + r] * qr[(long)jj * m + r];
+    for (int jj = ibgn + 2; jj <= iend - 2; jj += 2) acc += 2.0 * qf[(long)jj * m + r] * qr[(long)jj * m + r];
+    tot += lm[r] * acc;
+  }
+  const double s = block_sum(tot * a.ds[b] / 3.0, red);
+  if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = s;
+}
+
+// workgroup partials of sum x*y
+__global__ void __launch_bounds__(NTA) k_partial_dot(const double* __restrict__ x,
+                                                     const double* __restrict__ y, long n,
+                                                     double* __restrict__ part) {
+  __shared__ double red[NTA];
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA) acc += x[i] * y[i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y += x (n values)
+__global__ void __launch_bounds__(NTA) k_axpy1(const double* __restrict__ x, long n,
+                                               double* __restrict__ y) {
+  for (long i = blockIdx.x * (long)NTA + threadIdx.x; i < n; i += (long)gridDim.x * NTA) y[i] += x[i];
+}
+
+// Cell derivative of a |k|^2 gradient: for each cell parameter p (blockIdx.y),
+// workgroup partials of sum_k kg(k) * dksq_p(k), with dksq_p(k) = Gbz^T dGG_p Gbz
+// at the minimum-image wavevector Gbz of k (the make_dGsq form,
+// src/basis_mod.f90:1993-2014, on the grid).  dGG(dim, dim, ncp) Fortran order.
+__global__ void __launch_bounds__(NTA) k_dksq_dot(GeomArgs g, const double* __restrict__ dgg,
+                                                  const double* __restrict__ kg,
+                                                  double* __restrict__ part) {
+  __shared__ double red[NTA];
+  const int p = blockIdx.y, dim = g.dim;
+  const long mk = (long)(g.n1 / 2 + 1) * g.n2 * g.n3;
+  const double* d = dgg + (long)p * dim * dim;
+  double acc = 0.0;
+  for (long idx = blockIdx.x * (long)NTA + threadIdx.x; idx < mk; idx += (long)gridDim.x * NTA) {
+    int best[3];
+    (void)g_to_bz_point(g, idx, best);
+    double v = 0.0;
+    for (int jj = 0; jj < dim; ++jj)
+      for (int kk = 0; kk < dim; ++kk) v += (double)best[jj] * (double)best[kk] * d[jj + dim * kk];
+    acc += kg[idx] * v;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long)p * gridDim.x + blockIdx.x] = s;
 }
 
 // constant fill

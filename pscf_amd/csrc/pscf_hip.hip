@@ -113,12 +113,14 @@ using ZfftFn = void (*)(double2*, long, int, const double2*);
 using C2CFn = void (*)(double2*, long, const double2*, const double2*);
 
 struct PlaneSet {
+  int nt = 0, nt_c2c = 0;     // workgroup sizes (plane_nt, c2c_nt)
   StepPlaneFn a = nullptr, c = nullptr;
   R2CFn r2c = nullptr;
   C2RFn c2r = nullptr;
   C2CFn c2c_f = nullptr, c2c_i = nullptr;
 };
 struct ZSet {
+  int nt_half = 0, nt_full = 0;   // workgroup sizes (zcol_nt for N1/2+1 and N1 wide rows)
   StepZFn z = nullptr;
   ZfftFn half_f = nullptr, half_i = nullptr, full_f = nullptr, full_i = nullptr;
 };
@@ -126,6 +128,8 @@ struct ZSet {
 template <int A, int B>
 PlaneSet make_plane() {
   PlaneSet s;
+  s.nt = plane_nt(A, B);
+  s.nt_c2c = c2c_nt(A, B);
   s.a = k_step_a<A, B>;
   s.c = k_step_c<A, B>;
   s.r2c = k_plane_r2c<A, B>;
@@ -137,6 +141,8 @@ PlaneSet make_plane() {
 template <int A, int C>
 ZSet make_z() {
   ZSet s;
+  s.nt_half = zcol_nt(A / 2 + 1, C);
+  s.nt_full = zcol_nt(A, C);
   s.z = k_step_z<A, C>;
   s.half_f = k_zfft<A / 2 + 1, C, -1>;
   s.half_i = k_zfft<A / 2 + 1, C, +1>;
@@ -213,20 +219,22 @@ struct Grid {
 
   // real (nb arrays, stride m) -> half spectra (stride mk)
   void r2c(const double* in, double2* out, int nb, hipStream_t s) const {
-    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(NT), 0, s, in, m, out, mk, tw1, tw2);
-    if (n3 > 1) hipLaunchKernelGGL(zs->half_f, dim3(n2, nb), dim3(NT), 0, s, out, mk, n2, tw3);
+    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(pl->nt), 0, s, in, m, out, mk, tw1, tw2);
+    if (n3 > 1)
+      hipLaunchKernelGGL(zs->half_f, dim3(n2, nb), dim3(zs->nt_half), 0, s, out, mk, n2, tw3);
   }
   // half spectra -> real, in place on `in` for the z pass
   void c2r(double2* in, double* out, int nb, double scale, hipStream_t s) const {
-    if (n3 > 1) hipLaunchKernelGGL(zs->half_i, dim3(n2, nb), dim3(NT), 0, s, in, mk, n2, tw3);
-    hipLaunchKernelGGL(pl->c2r, dim3(n3, nb), dim3(NT), 0, s, in, mk, out, m, scale, tw1, tw2);
+    if (n3 > 1)
+      hipLaunchKernelGGL(zs->half_i, dim3(n2, nb), dim3(zs->nt_half), 0, s, in, mk, n2, tw3);
+    hipLaunchKernelGGL(pl->c2r, dim3(n3, nb), dim3(pl->nt), 0, s, in, mk, out, m, scale, tw1, tw2);
   }
   void c2c(double2* data, int sign, hipStream_t s) const {
-    hipLaunchKernelGGL(sign < 0 ? pl->c2c_f : pl->c2c_i, dim3(n3, 1), dim3(NT), 0, s, data, m, tw1,
-                       tw2);
+    hipLaunchKernelGGL(sign < 0 ? pl->c2c_f : pl->c2c_i, dim3(n3, 1), dim3(pl->nt_c2c), 0, s, data,
+                       m, tw1, tw2);
     if (n3 > 1)
-      hipLaunchKernelGGL(sign < 0 ? zs->full_f : zs->full_i, dim3(n2, 1), dim3(NT), 0, s, data, m,
-                         n2, tw3);
+      hipLaunchKernelGGL(sign < 0 ? zs->full_f : zs->full_i, dim3(n2, 1), dim3(zs->nt_full), 0, s,
+                         data, m, n2, tw3);
   }
 };
 
@@ -269,6 +277,33 @@ struct ChainDev {
   double length = 0;
   DevBuf<double> qf, qr;   // (ns+1) * m
   DevBuf<double> fac;      // per block: e1, e2 (m), k1, k2 (mk)
+};
+
+// Adjoint (reverse-mode) sweep state of a system: per chain four sweep tasks
+// (adjoint of q, recomputed q, adjoint of q-dagger, recomputed q-dagger),
+// their work and kept-field buffers, and the gradient accumulators.
+struct AdjState {
+  int ntask = 0, npair = 0;
+  bool with_k = false;
+  DevBuf<double> aslab;    // per chain: a_f, a_r slabs ((ns+1) m each)
+  DevBuf<double> gslab;    // per chain: sources g_f, g_r ((ns+1) m each)
+  DevBuf<double> keep;     // per task: U, V, W (m each)
+  DevBuf<double2> keepk;   // per task: 3 spectra (mk each), with_k only
+  DevBuf<double2> abc;     // per task: A, B, C (mk each)
+  DevBuf<double> q1;       // per task (m)
+  DevBuf<double> gblk;     // per chain and block: omega gradient (m)
+  DevBuf<double> kblk;     // per chain and block: |k|^2 gradient (mk), with_k only
+  DevBuf<StepTask> tasks;
+  DevBuf<int2> pairs;
+  DevBuf<double> part;     // reduction partials, and one result slot at the end
+  std::vector<StepTask> h_tasks;
+  std::vector<size_t> slab_off, blk_off;   // per chain: offset into aslab/gslab, into gblk/kblk (blocks)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~AdjState() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
 };
 
 struct System {
@@ -320,6 +355,7 @@ struct System {
     graph_exec = nullptr;
     graph = nullptr;
   }
+  std::unique_ptr<AdjState> adj;
   // stress work
   DevBuf<double> sbuf;     // real chunk
   DevBuf<double2> skbuf;   // spectra chunk
@@ -375,15 +411,18 @@ void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, h
   };
   for (int j = 0; j <= max_ns; ++j) {
     mark();
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1, g.tw2);
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask, 2), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
+                       g.tw1, g.tw2);
     if (j == max_ns) break;
     mark();
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n2, 0, g.tw3);
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 2), dim3(g.zs->nt_half), 0, s, d_tasks, j, g.n2,
+                       0, g.tw3);
     mark();
-    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, ntask, 2), dim3(NT), 0, s, d_tasks, j, g.n3, g.tw1,
-                       g.tw2);
+    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, ntask, 2), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
+                       g.tw1, g.tw2);
     mark();
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 1), dim3(NT), 0, s, d_tasks, j, g.n2, 1, g.tw3);
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 1), dim3(g.zs->nt_half), 0, s, d_tasks, j, g.n2,
+                       1, g.tw3);
   }
   mark();
   HIPCHK(hipGetLastError());
@@ -588,6 +627,7 @@ void startup(System& S, double chain_step) {
   S.ntask = ntask;
   S.max_ns = max_ns;
   S.drop_graph();
+  S.adj.reset();
   S.started = true;
   S.have_density = false;
 }
@@ -596,13 +636,16 @@ void startup(System& S, double chain_step) {
 
 // omega stars -> grid (no 1/M, :182-185), propagator factors per block
 // (make_propg), chemistry scalars for the device
+// (grid_in: S.omega_grid already holds the omega grids, star input unused)
 void density_pre(System& S, const double* omega, const double* phi_c, const double* mu_c,
-                 const double* phi_s, const double* mu_s, hipStream_t s) {
+                 const double* phi_s, const double* mu_s, hipStream_t s, bool grid_in = false) {
   REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
   const Grid& g = S.g;
-  const size_t nst = (size_t)S.nmon * S.nstar;
-  S.omega_in.upload(omega, nst, s);
-  basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p, s);
+  if (!grid_in) {
+    const size_t nst = (size_t)S.nmon * S.nstar;
+    S.omega_in.upload(omega, nst, s);
+    basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p, s);
+  }
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     for (int b = 0; b < ch.nblk; ++b) {
@@ -688,13 +731,15 @@ void density_post(System& S, double* rho, hipStream_t s) {
                        S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
   }
   const size_t nst = (size_t)S.nmon * S.nstar;
-  S.rho_out.alloc(nst);
-  grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p, s);
+  if (rho) {
+    S.rho_out.alloc(nst);
+    grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p, s);
+  }
   HIPCHK(hipGetLastError());
   S.h_Q.assign(S.nchain + S.nsolv, 0.0);
   HIPCHK(hipMemcpyAsync(S.h_Q.data(), S.qdev.p, sizeof(double) * (S.nchain + S.nsolv),
                         hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
+  if (rho) HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
 }
 
 // after the stream is synchronised: outputs and the grand-ensemble update (:202-209)
@@ -722,6 +767,299 @@ void density(System& S, const double* omega, double* rho, double* qout, double* 
   density_post(S, rho, s);
   S.st.sync();
   density_finish(S, qout, qsolv, phi_c, mu_c, phi_s, mu_s);
+}
+
+// density() on real-space grids held in device memory: omega_grid (M, N_monomer)
+// in, rho_grid (M, N_monomer) out, before any star projection.
+void density_grid(System& S, const double* omega_grid, double* rho_grid, double* qout, double* qsolv,
+                  double* phi_c, double* mu_c, double* phi_s, double* mu_s) {
+  REQUIRE(S.started, "system not ready (startup)");
+  hipStream_t s = S.st.s;
+  const size_t n = (size_t)S.nmon * S.g.m;
+  HIPCHK(hipMemcpyAsync(S.omega_grid.p, omega_grid, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  density_pre(S, nullptr, phi_c, mu_c, phi_s, mu_s, s, true);
+  density_sweeps(S, s);
+  density_post(S, nullptr, s);
+  HIPCHK(hipMemcpyAsync(rho_grid, S.rho_grid.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  S.st.sync();
+  density_finish(S, qout, qsolv, phi_c, mu_c, phi_s, mu_s);
+}
+
+// ---- reverse mode of density()
+//
+// For a scalar L(rho_grid, Q) with lam = dL/drho_grid (M, N_monomer) and dL/dQ
+// per chain, the adjoint fields of the two sweeps obey
+//   a_f(s) = g_f(s) + T_s a_f(s+1),   a_f(Ns) = g_f(Ns)      (backward in s)
+//   a_r(s) = g_r(s) + T_{s-1} a_r(s-1), a_r(0) = g_r(0)       (forward in s)
+// with the sources g_f(s) = c sum_b w_b(s) lam_mon(b) q_r(s) [+ Gam/M at Ns],
+// g_r(s) = c sum_b w_b(s) lam_mon(b) q_f(s) (w_b: Simpson weights times ds_b/3,
+// c = phi/(N Q) or e^mu/N, src/scf_mod.fp.f90:362-424) and Gam = dL/dQ
+// (canonical: minus sum lam rho_chain / Q, since rho ~ 1/Q).  The step
+// operator T (src/step_mod.f90:146-176) is symmetric, so the adjoint sweeps use
+// the forward step kernels.  Each adjoint sweep runs beside a recomputation of
+// the forward sweep it pairs with (same launches, steps in reverse order,
+// inputs re-read from the stored q slabs); every step keeps its convolution
+// fields, and the gradient kernels combine the pair after each step.
+
+constexpr int ADJ_NB = 512;   // workgroups of the adjoint reductions
+
+void adjoint_setup(System& S, bool with_k) {
+  if (S.adj && S.adj->with_k == with_k) return;
+  S.adj.reset();
+  auto A = std::make_unique<AdjState>();
+  const Grid& g = S.g;
+  A->with_k = with_k;
+  A->ntask = 4 * S.nchain;
+  A->npair = 2 * S.nchain;
+  size_t nslab = 0, nblk = 0;
+  for (int c = 0; c < S.nchain; ++c) {
+    A->slab_off.push_back(nslab);
+    A->blk_off.push_back(nblk);
+    nslab += (size_t)2 * (S.chains[c]->ns + 1) * g.m;
+    nblk += S.chains[c]->nblk;
+  }
+  A->aslab.alloc(nslab);
+  A->gslab.alloc(nslab);
+  A->keep.alloc((size_t)3 * g.m * A->ntask);
+  if (with_k) {
+    A->keepk.alloc((size_t)3 * g.mk * A->ntask);
+    A->kblk.alloc((size_t)g.mk * nblk);
+  }
+  A->abc.alloc((size_t)3 * g.mk * A->ntask);
+  A->q1.alloc((size_t)g.m * A->ntask);
+  A->gblk.alloc((size_t)g.m * nblk);
+  A->tasks.alloc(A->ntask);
+  A->pairs.alloc(A->npair);
+  A->part.alloc((size_t)ADJ_NB * MAXBLK + 1);
+  std::vector<StepTask> ht(A->ntask);
+  std::vector<int2> hp(A->npair);
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    const size_t sl = (size_t)(ch.ns + 1) * g.m;
+    double* af = A->aslab.p + A->slab_off[c];
+    double* ar = af + sl;
+    const double* gf = A->gslab.p + A->slab_off[c];
+    const double* gr = gf + sl;
+    // task order per chain: a_f, q_f recomputed, a_r, q_r recomputed
+    for (int k = 0; k < 4; ++k) {
+      const int ti = 4 * c + k;
+      StepTask t = S.h_tasks[2 * c + (k < 2 ? 0 : 1)];   // factors, blocks, ns of the chain
+      t.init_one = 0;
+      t.skip = 0;
+      if (k == 0 || k == 2) {       // adjoint sweeps, opposite direction to their q
+        t.dir = k == 0 ? -1 : 1;
+        t.slab = k == 0 ? af : ar;
+        t.src = k == 0 ? gf : gr;
+        t.add_src = 1;
+        t.rev = t.reload = t.nostore = 0;
+      } else {                      // recomputed forward sweeps
+        t.dir = k == 1 ? 1 : -1;
+        t.slab = k == 1 ? ch.qf.p : ch.qr.p;
+        t.src = nullptr;
+        t.add_src = 0;
+        t.rev = t.reload = t.nostore = 1;
+      }
+      t.A = A->abc.p + (size_t)(3 * ti) * g.mk;
+      t.B = t.A + g.mk;
+      t.C = t.B + g.mk;
+      t.q1 = A->q1.p + (size_t)ti * g.m;
+      t.keepU = A->keep.p + (size_t)(3 * ti) * g.m;
+      t.keepV = t.keepU + g.m;
+      t.keepW = t.keepV + g.m;
+      for (int q = 0; q < 3; ++q)
+        t.keepK[q] = with_k ? A->keepk.p + (size_t)(3 * ti + q) * g.mk : nullptr;
+      for (int b = 0; b < ch.nblk; ++b) {
+        t.ds[b] = ch.ds[b];
+        t.bsq[b] = S.kuhn[ch.mon[b]] * S.kuhn[ch.mon[b]];
+        t.grad[b] = A->gblk.p + (A->blk_off[c] + b) * g.m;
+        t.kgrad[b] = with_k ? A->kblk.p + (A->blk_off[c] + b) * g.mk : nullptr;
+      }
+      ht[ti] = t;
+    }
+    hp[2 * c] = make_int2(4 * c, 4 * c + 1);
+    hp[2 * c + 1] = make_int2(4 * c + 2, 4 * c + 3);
+  }
+  HIPCHK(hipMemcpyAsync(A->tasks.p, ht.data(), sizeof(StepTask) * A->ntask, hipMemcpyHostToDevice,
+                        S.st.s));
+  HIPCHK(hipMemcpyAsync(A->pairs.p, hp.data(), sizeof(int2) * A->npair, hipMemcpyHostToDevice,
+                        S.st.s));
+  S.st.sync();
+  A->h_tasks = ht;
+  S.adj = std::move(A);
+}
+
+// the adjoint sweeps with their gradient kernels (launch order as run_sweeps)
+void run_adj_sweeps(const System& S, hipStream_t s) {
+  const Grid& g = S.g;
+  const AdjState& A = *S.adj;
+  const int nt = A.ntask, max_ns = S.max_ns;
+  const int gb = grid_blocks(g.m), gk = grid_blocks(g.mk);
+  for (int j = 0; j <= max_ns; ++j) {
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
+                       g.tw2);
+    if (j >= 1)
+      hipLaunchKernelGGL(k_adj_grad, dim3(gb, A.npair), dim3(256), 0, s, A.tasks.p, A.pairs.p, j - 1,
+                         g.m);
+    if (j == max_ns) break;
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, nt, 2), dim3(g.zs->nt_half), 0, s, A.tasks.p, j, g.n2, 0,
+                       g.tw3);
+    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
+                       g.tw2);
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, nt, 1), dim3(g.zs->nt_half), 0, s, A.tasks.p, j, g.n2, 1,
+                       g.tw3);
+    if (A.with_k)
+      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk, A.npair), dim3(256), 0, s, A.tasks.p, A.pairs.p, j,
+                         g.n1, g.mk, (double)g.m);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// reduce np device partials to one host value (stream synchronised)
+double host_sum(System& S, const double* part, int np) {
+  double* slot = S.adj->part.p + S.adj->part.n - 1;
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, S.st.s, part, np, 1.0, slot);
+  double v = 0.0;
+  HIPCHK(hipMemcpyAsync(&v, slot, sizeof(double), hipMemcpyDeviceToHost, S.st.s));
+  S.st.sync();
+  return v;
+}
+
+// lam (M, N_monomer) and dQ (N_chain, may be null) on input; out: grad
+// (M, N_monomer, device) = dL/d omega_grid; with dgg (dim, dim, ncp) also
+// dL/d cell_param (ncp); with dlen also dL/d block length per chain and block
+// (block step count fixed; the chain length in the prefactor is the sum of
+// the block lengths).
+void density_adjoint(System& S, const double* lam, const double* dQ, double* grad, int ncp,
+                     const double* dgg, double* dcell, double* dlen) {
+  REQUIRE(S.have_density, "the adjoint requires a previous density()");
+  REQUIRE(S.nsolv == 0, "the adjoint does not support solvent species");
+  const bool with_k = dgg != nullptr || dlen != nullptr;
+  adjoint_setup(S, with_k);
+  AdjState& A = *S.adj;
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  const int nb = ADJ_NB;
+  REQUIRE(ncp <= 6, "at most 6 cell parameters");
+  double* part = A.part.p;
+  // Simpson sums T_b = sum_r lam_b rho-integrand of every block (for Gam and d/d ds)
+  std::vector<std::vector<double>> T(S.nchain);
+  std::vector<double> coef(S.nchain), gam(S.nchain);
+  std::vector<SimpsonArgs> sa(S.nchain);
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    SimpsonArgs& a = sa[c];
+    std::memset(&a, 0, sizeof a);
+    a.nblk = ch.nblk;
+    for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
+    for (int b = 0; b < ch.nblk; ++b) {
+      a.ds[b] = ch.ds[b];
+      a.monomer[b] = ch.mon[b];
+    }
+    a.chain_length = ch.length;
+    a.ensemble = S.ensemble;
+    hipLaunchKernelGGL(k_simpson_lam, dim3(nb, ch.nblk), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a, lam,
+                       part);
+    T[c].resize(ch.nblk);
+    for (int b = 0; b < ch.nblk; ++b) T[c][b] = host_sum(S, part + (size_t)b * nb, nb);
+    const double Q = S.h_Q[c];
+    coef[c] = S.ensemble == 1 ? std::exp(S.h_chem[2 * c + 1]) / ch.length
+                              : S.h_chem[2 * c] / (ch.length * Q);
+    double tsum = 0.0;
+    for (int b = 0; b < ch.nblk; ++b) tsum += T[c][b];
+    gam[c] = (dQ ? dQ[c] : 0.0) - (S.ensemble == 1 ? 0.0 : coef[c] * tsum / Q);
+  }
+  // sources, zeroed accumulators
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    AdjSrcArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nblk = ch.nblk;
+    a.ns = ch.ns;
+    for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
+    for (int b = 0; b < ch.nblk; ++b) {
+      a.ds[b] = ch.ds[b];
+      a.monomer[b] = ch.mon[b];
+    }
+    a.coef = coef[c];
+    a.endterm = gam[c] / (double)g.m;
+    double* gf = A.gslab.p + A.slab_off[c];
+    double* gr = gf + (size_t)(ch.ns + 1) * g.m;
+    hipLaunchKernelGGL(k_adj_sources, dim3(grid_blocks((long)(ch.ns + 1) * g.m)), dim3(NTA), 0, s,
+                       ch.qf.p, ch.qr.p, g.m, a, lam, gf, gr);
+  }
+  size_t nblk = 0;
+  for (int c = 0; c < S.nchain; ++c) nblk += S.chains[c]->nblk;
+  hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)(nblk * g.m))), dim3(NTA), 0, s, A.gblk.p,
+                     (long)(nblk * g.m), 0.0);
+  if (with_k)
+    hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)(nblk * g.mk))), dim3(NTA), 0, s, A.kblk.p,
+                       (long)(nblk * g.mk), 0.0);
+  // the sweeps (captured once as a graph)
+  if (S.use_graph) {
+    if (!A.exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_adj_sweeps(S, s);
+      HIPCHK(hipStreamEndCapture(s, &A.graph));
+      HIPCHK(hipGraphInstantiate(&A.exec, A.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(A.exec, s));
+  } else {
+    run_adj_sweeps(S, s);
+  }
+  // omega gradient per monomer: sum of its blocks
+  hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)S.nmon * g.m)), dim3(NTA), 0, s, grad,
+                     (long)S.nmon * g.m, 0.0);
+  for (int c = 0; c < S.nchain; ++c) {
+    ChainDev& ch = *S.chains[c];
+    for (int b = 0; b < ch.nblk; ++b)
+      hipLaunchKernelGGL(k_axpy1, dim3(grid_blocks(g.m)), dim3(NTA), 0, s,
+                         A.gblk.p + (A.blk_off[c] + b) * g.m, g.m, grad + (size_t)ch.mon[b] * g.m);
+  }
+  HIPCHK(hipGetLastError());
+  GeomArgs ga;
+  ga.n1 = g.n1;
+  ga.n2 = g.n2;
+  ga.n3 = g.n3;
+  ga.dim = g.dim;
+  for (int i = 0; i < 9; ++i) ga.gb[i] = S.gb[i];
+  for (int d = 0; d < 3; ++d) {
+    ga.gmin[d] = S.gmin[d];
+    ga.gext[d] = S.gext[d];
+  }
+  if (dgg && ncp > 0) {
+    DevBuf<double> d_dgg;
+    d_dgg.upload(dgg, (size_t)g.dim * g.dim * ncp, s);
+    for (int p = 0; p < ncp; ++p) dcell[p] = 0.0;
+    for (size_t k = 0; k < nblk; ++k) {
+      hipLaunchKernelGGL(k_dksq_dot, dim3(nb, ncp), dim3(NTA), 0, s, ga, d_dgg.p,
+                         A.kblk.p + k * g.mk, part);
+      for (int p = 0; p < ncp; ++p) dcell[p] += host_sum(S, part + (size_t)p * nb, nb);
+    }
+  }
+  if (dlen) {
+    // d/d ds_b = [sum_r omega grad_b + sum_k ksq kgrad_b + c T_b] / ds_b, and
+    // ds_b = L_b / (2 iblk_b) at a fixed step count, so d/dL_b = d/d ds_b * ds_b / L_b;
+    // plus the prefactor's 1/N, N = sum_b L_b: -c sum_b T_b / N
+    size_t o = 0;
+    for (int c = 0; c < S.nchain; ++c) {
+      ChainDev& ch = *S.chains[c];
+      double tsum = 0.0;
+      for (int b = 0; b < ch.nblk; ++b) tsum += T[c][b];
+      for (int b = 0; b < ch.nblk; ++b, ++o) {
+        const double* gb_ = A.gblk.p + (A.blk_off[c] + b) * g.m;
+        hipLaunchKernelGGL(k_partial_dot, dim3(nb), dim3(NTA), 0, s, gb_,
+                           S.omega_grid.p + (size_t)ch.mon[b] * g.m, g.m, part);
+        const double t1 = host_sum(S, part, nb);
+        hipLaunchKernelGGL(k_partial_dot, dim3(nb), dim3(NTA), 0, s, A.kblk.p + (A.blk_off[c] + b) * g.mk,
+                           S.ksq.p, g.mk, part);
+        const double t2 = host_sum(S, part, nb);
+        const double dds = (t1 + t2 + coef[c] * T[c][b]) / ch.ds[b];
+        const double L = S.blocks[c][b].second;
+        dlen[o] = dds * ch.ds[b] / L - coef[c] * tsum / ch.length;
+      }
+    }
+  }
+  S.st.sync();
 }
 
 // ---- scf_stress (src/scf_mod.fp.f90:451-550)
@@ -1480,6 +1818,47 @@ int pscf_density(int64_t sys, const double* omega, double* rho, double* qout, do
   return guarded([&] {
     density(*from_handle<System>(sys, "system"), omega, rho, qout, q_solvent, phi_chain, mu_chain,
             phi_solvent, mu_solvent);
+  });
+}
+
+int pscf_density_grid_dev(int64_t sys, const double* omega_grid, double* rho_grid, double* qout,
+                          double* q_solvent, double* phi_chain, double* mu_chain,
+                          double* phi_solvent, double* mu_solvent) {
+  return guarded([&] {
+    REQUIRE(omega_grid && rho_grid, "null grid pointer");
+    density_grid(*from_handle<System>(sys, "system"), omega_grid, rho_grid, qout, q_solvent,
+                 phi_chain, mu_chain, phi_solvent, mu_solvent);
+  });
+}
+
+int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const double* dL_dQ,
+                             double* grad_omega_grid, int n_cell_param, const double* dGG,
+                             double* dL_dcell, double* dL_dlength) {
+  return guarded([&] {
+    REQUIRE(lam_rho_grid && grad_omega_grid, "null grid pointer");
+    REQUIRE(!dGG || dL_dcell, "dGG given without an output for the cell gradient");
+    density_adjoint(*from_handle<System>(sys, "system"), lam_rho_grid, dL_dQ, grad_omega_grid,
+                    n_cell_param, dGG, dL_dcell, dL_dlength);
+  });
+}
+
+int pscf_basis_to_grid_dev(int64_t sys, const double* basis, int nf, double* grid) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_basis && S.have_cell, "system not ready (basis, cell)");
+    REQUIRE(nf >= 1 && basis && grid, "invalid arguments");
+    basis_to_grid(S, basis, nf, grid, S.st.s);
+    S.st.sync();
+  });
+}
+
+int pscf_grid_to_basis_dev(int64_t sys, const double* grid, int nf, double scale, double* basis) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_basis && S.have_cell, "system not ready (basis, cell)");
+    REQUIRE(nf >= 1 && basis && grid, "invalid arguments");
+    grid_to_basis(S, grid, nf, scale, basis, S.st.s);
+    S.st.sync();
   });
 }
 

@@ -52,6 +52,7 @@ def get_lib(kind):
             import subprocess
             subprocess.run([os.path.join(EMU_DIR, "build_emu.sh")], check=True)
         L = Lib(EMU_LIB)
+        L.tensor_device = "cpu"      # emulated device memory is host memory
     else:
         L = Lib(DEFAULT_LIB)
         assert L.device_count() >= 1, "no HIP device visible"

@@ -23,6 +23,8 @@
 
 struct double2 { double x, y; };
 inline double2 make_double2(double a, double b) { return double2{a, b}; }
+struct int2 { int x, y; };
+inline int2 make_int2(int a, int b) { return int2{a, b}; }
 
 struct dim3 {
   unsigned x, y, z;

@@ -65,6 +65,19 @@ def cases():
     C["gyr64"] = (diblock(0.36, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[16.0],
                           ngrid=[64, 64, 64], chain_step=0.5, group_name="I a -3 d"),
                   20261201, dict(grids=False, step=False, nstar_seed=21))
+    # tables for the batched-sweep workload (N=100 diblocks, ds=1): lamellae in a
+    # tetragonal cell on 32^3, hexagonal cylinders (3D cell) on 48^3, BCC on 64^3
+    C["lam_tet32"] = (diblock(0.45, 100.0, 20.0, dim=3, crystal_system="tetragonal",
+                              cell_param=[16.0, 16.0], ngrid=[32, 32, 32], chain_step=1.0,
+                              group_name="P 4/m m m"),
+                      20261203, dict(grids=False, step=False, nstar_seed=11))
+    C["hex48"] = (diblock(0.3, 100.0, 20.0, dim=3, crystal_system="hexagonal",
+                          cell_param=[19.0, 19.0], ngrid=[48, 48, 48], chain_step=1.0,
+                          group_name="P 6/m m m"),
+                  20261204, dict(grids=False, step=False, nstar_seed=11))
+    C["bcc64"] = (diblock(0.25, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[18.0],
+                          ngrid=[64, 64, 64], chain_step=1.0, group_name="I m -3 m"),
+                  20261205, dict(grids=False, step=False, nstar_seed=11))
     return C
 
 

@@ -1,0 +1,231 @@
+"""Reverse-mode derivatives of density() (pscf_amd.adjoint over
+pscf_density_adjoint_dev) against central finite differences of the same
+forward, and against the reference's own identities:
+
+* d ln Q / d omega_a(r) = -(N / (M phi)) rho_a(r)   (continuous-s identity)
+* d/d(cell) [-(phi/N) ln Q] = scf_stress             (src/scf_mod.fp.f90:451-550)
+
+The adjoint differentiates the discrete map exactly, so finite differences
+agree to rounding and truncation error (tolerance 1e-6 relative).  The
+identities hold for the continuous contour only; their gap must shrink at
+the contour discretisation's order as ds is halved.
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN, get_lib, load_golden
+
+sys.path.insert(0, GOLDEN)
+from make_golden import cases  # noqa: E402
+
+from pscf_amd import adjoint, batch, iterate  # noqa: E402
+from pscf_amd.system import System  # noqa: E402
+
+FD_TOL = 1e-6
+
+
+def _system(name, step=None):
+    if name == "blend_nosolv":     # ABA triblock + A homopolymer, 3 + 1 blocks, two chains
+        s = copy.deepcopy(cases()["blend12"][0])
+        s = System(kuhn=s.kuhn, chains=s.chains, phi_chain=[0.85, 0.15], chi=s.chi, dim=s.dim,
+                   crystal_system=s.crystal_system, cell_param=s.cell_param, ngrid=s.ngrid,
+                   chain_step=s.chain_step, group_name=s.group_name)
+        g = load_golden("blend12")
+    else:
+        s = copy.deepcopy(cases()[name][0])
+        g = load_golden(name)
+    if step is not None:
+        s.chain_step = step
+    return s, g
+
+
+def _make(name, backend, step=None):
+    lib = get_lib(backend)
+    s, g = _system(name, step)
+    S = batch.make_system(s, g, lib=lib)
+    dev = lib.tensor_device
+    omg = adjoint.stars_to_grid(S, torch.tensor(g["omega"], dtype=torch.float64, device=dev))
+    return S, s, g, omg.detach(), dev
+
+
+def _fd(f, h):
+    return (f(h) - f(-h)) / (2.0 * h)
+
+
+# contour steps chosen so that no block length sits on a rounding boundary of
+# its step count (hex24's own 0.02 puts 0.3/0.02/2 + 0.5 exactly at 8)
+@pytest.mark.parametrize("name,step", [("bcc8", None), ("hex24", 0.025), ("blend_nosolv", 0.09)])
+def test_gradients_match_central_fd(backend, name, step):
+    S, s, g, omg, dev = _make(name, backend, step)
+    rng = np.random.default_rng(11)
+    tgt = torch.tensor(rng.normal(0.3, 0.05, omg.shape[1:]), device=dev)
+
+    def loss(om, cp, L):
+        rho, Q = adjoint.density(S, om, cp, L)
+        out = ((rho[0] - tgt) ** 2).sum()
+        for c in range(s.N_chain):
+            out = out + 0.3 * (c + 1) * torch.log(Q[c])
+        return out
+
+    cp0 = torch.tensor(np.array(s.cell_param, dtype=np.float64))
+    L0 = adjoint.block_lengths(s)
+    om = omg.clone().requires_grad_(True)
+    cp = cp0.clone().requires_grad_(True)
+    Lb = L0.clone().requires_grad_(True)
+    loss(om, cp, Lb).backward()
+
+    d = torch.tensor(rng.normal(size=omg.shape), device=dev)   # arbitrary, not symmetric
+    with torch.no_grad():
+        fd = _fd(lambda h: loss(omg + h * d, cp0, L0).item(), 1e-6)
+    ad = (om.grad * d).sum().item()
+    assert abs(ad - fd) <= FD_TOL * abs(fd), ("omega", ad, fd)
+
+    for p in range(len(cp0)):
+        e = torch.zeros_like(cp0)
+        e[p] = 1.0
+        h = 1e-6 * cp0[p].item()
+        with torch.no_grad():
+            fd = _fd(lambda t: loss(omg, cp0 + t * e, L0).item(), h)
+        assert abs(cp.grad[p].item() - fd) <= FD_TOL * abs(fd), ("cell", p, cp.grad[p].item(), fd)
+
+    for b in range(len(L0)):
+        e = torch.zeros_like(L0)
+        e[b] = 1.0
+        h = 1e-6 * L0[b].item()
+        with torch.no_grad():
+            fd = _fd(lambda t: loss(omg, cp0, L0 + t * e).item(), h)
+        assert abs(Lb.grad[b].item() - fd) <= FD_TOL * max(abs(fd), 1e-3), ("length", b,
+                                                                           Lb.grad[b].item(), fd)
+
+
+def test_free_energy_gradient_matches_fd(backend):
+    S, s, g, omg, dev = _make("bcc8", backend)
+    rng = np.random.default_rng(12)
+
+    def F(om):
+        rho, Q = adjoint.density(S, om)
+        return adjoint.free_energy(S, om, rho, Q)
+
+    om = omg.clone().requires_grad_(True)
+    F(om).backward()
+    d = torch.tensor(rng.normal(size=omg.shape), device=dev)
+    with torch.no_grad():
+        fd = _fd(lambda h: F(omg + h * d).item(), 1e-6)
+    ad = (om.grad * d).sum().item()
+    assert abs(ad - fd) <= FD_TOL * abs(fd), (ad, fd)
+
+
+def test_star_free_energy_value_and_gradient(backend):
+    """F of star coefficients through stars -> grid -> density -> stars:
+    value equal to the reference's free_energy at the same omega, gradient
+    against central FD along random star directions."""
+    S, s, g, omg, dev = _make("bcc8", backend)
+
+    def F(ws):
+        rho, Q = adjoint.density(S, adjoint.stars_to_grid(S, ws))
+        return adjoint.free_energy_stars(S, ws, adjoint.grid_to_stars(S, rho), Q)
+
+    w0 = torch.tensor(g["omega"], dtype=torch.float64, device=dev)
+    w = w0.clone().requires_grad_(True)
+    f = F(w)
+    f.backward()
+    rho_st, q, _ = S.density(g["omega"])
+    S.mu_phi_chain(q)
+    f_ref, _ = S.free_energy(rho_st, g["omega"])
+    assert abs(f.item() - f_ref) < 1e-12 * max(1.0, abs(f_ref)), (f.item(), f_ref)
+    rng = np.random.default_rng(13)
+    for _ in range(2):
+        d = torch.tensor(rng.normal(size=w0.shape), device=dev)
+        with torch.no_grad():
+            fd = _fd(lambda h: F(w0 + h * d).item(), 1e-6)
+        ad = (w.grad * d).sum().item()
+        assert abs(ad - fd) <= FD_TOL * abs(fd), (ad, fd)
+
+
+def test_grid_density_matches_star_density(backend):
+    S, s, g, omg, dev = _make("p213_12", backend)
+    with torch.no_grad():
+        rho, Q = adjoint.density(S, omg)
+        rho_st = adjoint.grid_to_stars(S, rho).cpu().numpy()
+    rho_ref, q_ref, _ = S.density(g["omega"])
+    np.testing.assert_allclose(Q.cpu().numpy(), q_ref, rtol=1e-13)
+    np.testing.assert_allclose(rho_st, rho_ref, rtol=0, atol=1e-12 * np.abs(rho_ref).max())
+
+
+@pytest.mark.parametrize("which", ["omega", "stress"])
+def test_identities_converge_with_ds(backend, which):
+    gaps = []
+    for step in (0.02, 0.01):
+        S, s, g, omg, dev = _make("lam32", backend, step=step)
+        om = omg.clone().requires_grad_(True)
+        cp = torch.tensor(np.array(s.cell_param, dtype=np.float64), requires_grad=True)
+        rho, Q = adjoint.density(S, om, cp)
+        N = sum(L for _, L in s.chains[0])
+        phi = s.phi_chain[0]
+        (-(phi / N) * torch.log(Q[0])).backward()
+        if which == "omega":
+            ident = rho.detach() / omg[0].numel()       # -(phi/N) * (-(N/(M phi)) rho)
+            gaps.append(((om.grad - ident).abs().max() / ident.abs().max()).item())
+        else:
+            _, _, dGG = iterate.unit_cell(S.lib, s.dim, s.crystal_system, s.cell_param)
+            dgsq = iterate.make_dgsq(S.lib, S.basis, dGG, len(s.cell_param))
+            S.density(g["omega"])
+            st = S.scf_stress(dgsq)
+            gaps.append(float(np.abs(cp.grad.numpy() - st).max() / np.abs(st).max()))
+    # third-order contour convergence measured here: each halving cuts the gap ~8x
+    assert gaps[1] < gaps[0] / 4.0, gaps
+    assert gaps[1] < (5e-4 if which == "omega" else 1e-3), gaps
+
+
+def test_backward_after_new_forward_raises(backend):
+    S, s, g, omg, dev = _make("bcc8", backend)
+    om = omg.clone().requires_grad_(True)
+    rho, Q = adjoint.density(S, om)
+    with torch.no_grad():
+        adjoint.density(S, omg)
+    with pytest.raises(RuntimeError, match="call backward first"):
+        rho.sum().backward()
+
+
+@pytest.mark.gpu
+def test_gyroid64_loss_gradients_gpu():
+    """The gyroid 64^3 system (N=100, f=0.36, chi N=20, a=16, ds=0.5): loss
+    sum_r (phi_A(r) - phi_t(r))^2 with phi_t the first 20 gyroid stars around
+    0.36 (amplitude 0.1); gradients in omega, a and f_A against central FD."""
+    lib = get_lib("gpu")
+    s, g = _system("gyr64")
+    S = batch.make_system(s, g, lib=lib)
+    dev = lib.tensor_device
+    rng = np.random.default_rng(2026)
+    tst = np.zeros((1, S.N_star))
+    tst[0, 0] = 0.36
+    tst[0, 1:20] = rng.normal(size=19)
+    tst[0, 1:20] *= 0.1 / np.abs(tst[0, 1:20]).max()
+    phi_t = adjoint.stars_to_grid(S, torch.tensor(tst, device=dev))[0].detach()
+    omg = adjoint.stars_to_grid(S, torch.tensor(g["omega"], device=dev)).detach()
+    N = 100.0
+
+    def loss(om, cp, fA):
+        L = torch.stack([fA * N, (1.0 - fA) * N])
+        rho, Q = adjoint.density(S, om, cp, L)
+        return ((rho[0] - phi_t) ** 2).sum()
+
+    cp0 = torch.tensor([16.0], dtype=torch.float64)
+    f0 = torch.tensor(0.36, dtype=torch.float64)
+    om = omg.clone().requires_grad_(True)
+    cp = cp0.clone().requires_grad_(True)
+    fA = f0.clone().requires_grad_(True)
+    loss(om, cp, fA).backward()
+    d = torch.tensor(rng.normal(size=omg.shape), device=dev)
+    with torch.no_grad():
+        fd_w = _fd(lambda h: loss(omg + h * d, cp0, f0).item(), 1e-6)
+        fd_a = _fd(lambda h: loss(omg, cp0 + h, f0).item(), 16e-6)
+        fd_f = _fd(lambda h: loss(omg, cp0, f0 + h).item(), 1e-6)
+    ad_w = (om.grad * d).sum().item()
+    assert abs(ad_w - fd_w) <= FD_TOL * abs(fd_w), (ad_w, fd_w)
+    assert abs(cp.grad.item() - fd_a) <= FD_TOL * abs(fd_a), (cp.grad.item(), fd_a)
+    assert abs(fA.grad.item() - fd_f) <= FD_TOL * abs(fd_f), (fA.grad.item(), fd_f)

@@ -25,7 +25,7 @@ import pscf_oracle as O  # noqa: E402
 from pscf_amd import scf  # noqa: E402
 
 SMALL = ["bcc8", "lam32", "hex24", "p213_12", "blend12"]
-LARGE = ["bcc32", "gyr64"]
+LARGE = ["bcc32", "gyr64", "lam_tet32", "hex48", "bcc64"]
 
 
 def _cases(backend):
