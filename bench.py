@@ -29,6 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by the same counters on a plain copy of known size); the summary and
+# its commands are in profiles/.  None until measured.
+TRAFFIC = {}
 WORKLOAD = "gyroid_Ia-3d_64^3_N100_f0.36_chiN20_ds0.5_a16_AM_domain"
 
 
@@ -43,11 +47,16 @@ def gyroid_system():
                    ngrid=[64, 64, 64], chain_step=0.5, group_name="I a -3 d")
 
 
-def step_bytes(M, Mk):
-    """Algorithmic HBM bytes of one contour step of one sweep (streaming-FFT
-    model: every 3D transform reads its input and writes its output once,
-    pointwise factors fused into the adjacent transform)."""
-    return 96.0 * M + 120.0 * Mk
+def kernel_bytes(M, Mk):
+    """Algorithmic HBM bytes per launch of each contour-step kernel for ONE
+    sweep (every array the kernel must read or write, once: FP64 real fields
+    8 B/point, half spectra 16 B/point, factors e1, e2 (M) and k1, k2 (Mk)):
+      step_a : C (Mk) in, q1, e2, e1 (M) in, q (M) out, A, B (Mk) out
+      step_z0: A, B (Mk) in and out, k1, k2 (Mk) in
+      step_c : A, B (Mk) in, e1 (M) in, q1 (M) out, C (Mk) out
+      step_z1: C (Mk) in and out, k2 (Mk) in"""
+    return {"step_a": 32.0 * M + 48.0 * Mk, "step_z0": 80.0 * Mk,
+            "step_c": 16.0 * M + 48.0 * Mk, "step_z1": 40.0 * Mk}
 
 
 def cpu_baseline(g, n_gpu_iter, workdir):
@@ -90,13 +99,172 @@ def cpu_baseline(g, n_gpu_iter, workdir):
                       "make_basis)" % (max_itr, scf_time, itr, t_iter, n_gpu_iter, wall)}
 
 
+# ---------------------------------------------------------------- batched leg
+# 64 AB diblock melts (N=100, ds=1) with chi N ~ U[12, 30], f_A ~ U[0.2, 0.5]
+# and a phase drawn per instance: lamellae in a tetragonal cell on 32^3,
+# hexagonal cylinders (3D cell) on 48^3, BCC or gyroid on 64^3.  Each runs
+# its own Anderson history (fixed cell) to residual 1e-8.
+BATCH_PHASES = {
+    "lam": ("lam_tet32", "tetragonal", [16.0, 16.0], [32, 32, 32], "P 4/m m m"),
+    "hex": ("hex48", "hexagonal", [19.0, 19.0], [48, 48, 48], "P 6/m m m"),
+    "bcc": ("bcc64", "cubic", [18.0], [64, 64, 64], "I m -3 m"),
+    "gyr": ("gyr64", "cubic", [36.0], [64, 64, 64], "I a -3 d"),
+}
+BATCH_SEED = 20261203
+
+
+def batch_instances(n=64, seed=BATCH_SEED):
+    from pscf_amd.system import diblock
+    rng = np.random.default_rng(seed)
+    tables = {}
+    out = []
+    for _ in range(n):
+        ph = ["lam", "hex", "bcc", "gyr"][int(rng.integers(4))]
+        chiN = float(rng.uniform(12.0, 30.0))
+        fA = float(rng.uniform(0.2, 0.5))
+        tname, cs, cell, grid, group = BATCH_PHASES[ph]
+        if tname not in tables:
+            d = np.load(os.path.join(ROOT, "tests", "golden", tname + ".npz"))
+            tables[tname] = {k: d[k] for k in d.files}
+        g = tables[tname]
+        sysm = diblock(fA, 100.0, chiN, dim=3, crystal_system=cs, cell_param=cell, ngrid=grid,
+                       chain_step=1.0, group_name=group)
+        nstar = len(g["star_count"])
+        om = np.zeros((2, nstar))
+        om[:, 1:10] = rng.normal(0.0, 0.5 * chiN / 100.0, size=(2, 9))
+        out.append((ph, sysm, g, om))
+    return out
+
+
+def batch_leg(L, n_inst=64, n_loop=16, error_max=1e-8, max_itr=500):
+    """Converged solves/s of the batched solver over all instances, and of a
+    per-instance loop (the plain single-system path) over every (n_inst /
+    n_loop)-th instance."""
+    import torch
+    from pscf_amd import batch, iterate
+    insts = batch_instances(n_inst)
+    B = batch.BatchSolver(n_hist=20, domain=False, error_max=error_max, max_itr=max_itr, lib=L)
+    for ph, sysm, g, om in insts:
+        B.add(batch.make_system(sysm, g, lib=L), om)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.solve()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    res = [B.result(i) for i in range(n_inst)]
+    del B
+    conv_b = sum(r.converge for r in res)
+    stride = max(1, n_inst // n_loop)
+    sub = list(range(0, n_inst, stride))[:n_loop]
+    t_loop = 0.0
+    conv_l = 0
+    same = 0
+    for i in sub:
+        ph, sysm, g, om = insts[i]
+        S = batch.make_system(sysm, g, lib=L)
+        am = iterate.AndersonMixing(S, n_hist=20, domain=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = am.run(om, error_max=error_max, max_itr=max_itr)
+        torch.cuda.synchronize()
+        t_loop += time.perf_counter() - t0
+        conv_l += r.converge
+        same += (r.itr == res[i].itr and r.converge == res[i].converge)
+        del am, S
+    phases = {}
+    for ph, *_ in insts:
+        phases[ph] = phases.get(ph, 0) + 1
+    v_b = conv_b / t_batch
+    v_l = conv_l / t_loop if t_loop > 0 else None
+    return {"metric": "batched converged SCFT solves/s", "value": v_b, "unit": "solves/s",
+            "instances": n_inst, "converged": conv_b, "phases": phases,
+            "seconds": t_batch, "iterations": [r.itr for r in res],
+            "loop_value": v_l, "loop_instances": len(sub), "loop_converged": conv_l,
+            "loop_seconds": t_loop, "speedup_vs_loop": (v_b / v_l) if v_l else None,
+            "loop_matches_batch": same,
+            "workload": "64 AB diblocks N=100 ds=1, chiN~U[12,30], fA~U[0.2,0.5], "
+                        "phases lam(32^3)/hex(48^3)/bcc,gyr(64^3), AM n_hist=20, fixed cell, "
+                        "residual 1e-8, seed %d" % BATCH_SEED}
+
+
+# --------------------------------------------------------------- adjoint leg
+def adjoint_leg(L, g, omega, cell_a, t_solve, reps=3):
+    """Cost of the reverse-mode gradient of L = sum_r (phi_A(r) - phi_t(r))^2
+    (phi_t: first 20 gyroid stars, amplitude 0.1 around 0.36, seed 2026) with
+    respect to omega(r), the cell a and f_A at the converged gyroid."""
+    import torch
+    from pscf_amd import adjoint, batch
+    sysm = gyroid_system()
+    sysm.cell_param = [float(cell_a)]
+    S = batch.make_system(sysm, g, lib=L)
+    dev = "cuda"
+    rng = np.random.default_rng(2026)
+    tst = np.zeros((1, S.N_star))
+    tst[0, 0] = 0.36
+    tst[0, 1:20] = rng.normal(size=19)
+    tst[0, 1:20] *= 0.1 / np.abs(tst[0, 1:20]).max()
+    phi_t = adjoint.stars_to_grid(S, torch.tensor(tst, device=dev))[0].detach()
+    omg = adjoint.stars_to_grid(S, torch.tensor(omega, device=dev)).detach()
+    fwd, bwd = [], []
+    for _ in range(reps + 1):
+        om = omg.clone().requires_grad_(True)
+        cp = torch.tensor([float(cell_a)], dtype=torch.float64, requires_grad=True)
+        fA = torch.tensor(0.36, dtype=torch.float64, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rho, Q = adjoint.density(S, om, cp, torch.stack([fA * 100.0, (1.0 - fA) * 100.0]))
+        loss = ((rho[0] - phi_t) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f_ms = float(np.median(fwd[1:])) * 1e3
+    b_ms = float(np.median(bwd[1:])) * 1e3
+    return {"metric": "adjoint gradient cost", "forward_density_ms": f_ms, "backward_ms": b_ms,
+            "gradient_over_density": (f_ms + b_ms) / f_ms,
+            "gradient_over_solve": (f_ms + b_ms) * 1e-3 / t_solve,
+            "wrt": ["omega(r)", "cell a", "f_A"],
+            "loss": "sum_r (phi_A - phi_t)^2, gyroid 64^3 at the converged state"}
+
+
+def traffic_probe():
+    import torch
+    from pscf_amd import iterate, scf
+    from pscf_amd._lib import lib
+    L = lib()
+    g = load_workload()
+    sysm = gyroid_system()
+    _, G0, _ = iterate.unit_cell(L, 3, "cubic", [16.0])
+    S = scf.SCFSystem(sysm, scf.Basis.from_tables(g), G0, lib=L)
+    S.use_graph(False)
+    for _ in range(2):
+        S.density(g["omega"])
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("traffic probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient-cost leg")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run two eagerly launched density() calls of the workload and a "
+                         "256 MiB device copy (the HBM counter runs under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent solves shard across GPUs without communication; see DESIGN.md)")
@@ -133,11 +301,29 @@ def main():
 
     M = 64 ** 3
     Mk = 33 * 64 * 64
-    # slot 4: milliseconds of whole sweeps and the contour steps they advanced
+    # whole captured sweeps (slot 4): milliseconds and contour steps advanced
     steps_timed = int(n[4])
     t_step_ms = float(ms[4]) / max(steps_timed, 1)
-    bytes_step = 2 * step_bytes(M, Mk)           # qf and qr sweeps advance together
-    achieved = bytes_step / (t_step_ms * 1e-3) / 1e9
+    # per-kernel durations: HIP events around every step kernel on the system
+    # stream, over two density() calls at the converged field
+    S.profile_sweeps(2)
+    for _ in range(2):
+        S.density(r.omega)
+    torch.cuda.synchronize()
+    kms, kn = S.profile_sweeps(0)
+    names = ["step_a", "step_z0", "step_c", "step_z1"]
+    kavg_us = {nm: float(kms[i]) / max(int(kn[i]), 1) * 1e3 for i, nm in enumerate(names)}
+    ktot = {nm: float(kms[i]) for i, nm in enumerate(names)}
+    kb = {nm: 2 * v for nm, v in kernel_bytes(M, Mk).items()}   # qf and qr sweeps per launch
+    dom = max(names, key=lambda nm: ktot[nm])
+    achieved = kb[dom] / (kavg_us[dom] * 1e-6) / 1e9
+    bytes_step = sum(kb.values())
+
+    extra = {}
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_leg(L, g, r.omega, r.cell_param[0], elapsed / args.steps)
+    if not args.no_batch:
+        extra["batch"] = batch_leg(L)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -164,13 +350,18 @@ def main():
                    "iterations_per_solve": iters, "converged": [x.converge for x in results],
                    "final_cell_a": float(r.cell_param[0]), "f_helmholtz": r.f_helmholtz,
                    "ms_per_iteration": elapsed / sum(iters) * 1e3},
-        "roofline": {"bound": "hbm", "kernel": "contour step (4 fused passes, both sweeps)",
+        "roofline": {"bound": "hbm", "kernel": "k_step_" + dom[5:] + "<64,64> (both sweeps)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "us_per_step": t_step_ms * 1e3, "steps_timed": steps_timed,
-                     "bytes_per_step": bytes_step},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom),
+                     "bytes_per_launch": kb[dom], "kernel_avg_us": kavg_us,
+                     "kernel_achieved_gbs": {nm: kb[nm] / (kavg_us[nm] * 1e-6) / 1e9
+                                             for nm in names},
+                     "step_us_graph": t_step_ms * 1e3, "steps_timed": steps_timed,
+                     "step_bytes": bytes_step,
+                     "step_achieved_gbs": bytes_step / (t_step_ms * 1e-3) / 1e9},
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

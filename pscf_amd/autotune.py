@@ -1,0 +1,163 @@
+"""Autotuner of the contour-step kernel plans (pscf_amd/csrc/tuned.hpp).
+
+The plan of a grid length n is (max_radix, xpad, zsplit): the largest Stockham
+radix of its transforms, the LDS padding of the XY-plane rows and the number
+of workgroups per ky row of the z-column step kernel.  They are compile-time
+parameters, so the tuner builds one library per candidate plan (overriding
+the table for every length), times the captured contour sweeps of a
+synthetic system on each candidate and grid length on the device, and writes
+the fastest plan per length into tuned.hpp.
+
+    python -m pscf_amd.autotune build              # compile the candidates (CPU)
+    python -m pscf_amd.autotune measure [--sizes 32,48,64,96]   # time them (GPU)
+    python -m pscf_amd.autotune write              # regenerate csrc/tuned.hpp
+
+Timing workload per length n: an AB diblock (N=100, f=0.3, chi N=20, ds=1,
+100 contour steps) in a cubic cell on an n^3 grid with a seeded random omega,
+both sweeps per launch, graph-captured, HIP events around every sweep.
+"""
+import argparse
+import ctypes
+import itertools
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNE_DIR = os.path.join(HERE, "_tune")
+RESULTS = os.path.join(TUNE_DIR, "results.json")
+TUNED_HPP = os.path.join(HERE, "csrc", "tuned.hpp")
+
+RADIX = (8, 4)
+XPAD = (0, 1)
+ZSPLIT = (1, 2, 3)
+
+
+def candidates():
+    return [dict(max_radix=r, xpad=p, zsplit=z) for r, p, z in itertools.product(RADIX, XPAD, ZSPLIT)]
+
+
+def variant_name(c):
+    return "r%d_p%d_z%d" % (c["max_radix"], c["xpad"], c["zsplit"])
+
+
+def build(jobs=4):
+    from .build import build_lib
+    os.makedirs(TUNE_DIR, exist_ok=True)
+
+    def one(c):
+        out = os.path.join(TUNE_DIR, "lib_%s.so" % variant_name(c))
+        build_lib(defines={"PSCF_MAX_RADIX": c["max_radix"], "PSCF_XPAD": c["xpad"],
+                           "PSCF_ZSPLIT": c["zsplit"]}, out=out)
+        return out
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(one, candidates()):
+            print("built", out, flush=True)
+
+
+def _trivial_basis():
+    from .scf import Basis
+    return Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
+
+
+def time_sweeps(lib, n, reps=5, seed=7):
+    """Microseconds per contour step (both sweeps) of the synthetic n^3 system."""
+    import torch
+    from .iterate import unit_cell
+    from .scf import SCFSystem
+    from .system import diblock
+    s = diblock(0.3, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[float(n) / 3.0],
+                ngrid=[n, n, n], chain_step=1.0, group_name="P 1")
+    _, G, _ = unit_cell(lib, 3, "cubic", s.cell_param)
+    S = SCFSystem(s, _trivial_basis(), G, lib=lib)
+    rng = np.random.default_rng(seed)
+    dev = lib.tensor_device
+    om = torch.tensor(0.2 + 0.1 * rng.standard_normal((2, n, n, n)), device=dev)
+    rho = torch.empty_like(om)
+    q = np.zeros(1)
+    qs = np.zeros(1)
+    from ._lib import dptr
+
+    def dens():
+        lib.call("pscf_density_grid_dev", S.h, ctypes.c_void_p(om.data_ptr()),
+                 ctypes.c_void_p(rho.data_ptr()), dptr(q), dptr(qs), dptr(S.phi_chain),
+                 dptr(S.mu_chain), dptr(S.phi_solvent), dptr(S.mu_solvent))
+
+    if dev == "cuda":
+        torch.cuda.synchronize()
+    dens()
+    S.profile_sweeps(1)
+    for _ in range(reps):
+        dens()
+    ms, cnt = S.profile_sweeps(0)
+    return float(ms[4]) / max(int(cnt[4]), 1) * 1e3, float(rho.sum().item())
+
+
+def measure(sizes, reps=5, results=RESULTS):
+    from ._lib import Lib
+    res = {}
+    if os.path.exists(results):
+        res = json.load(open(results))
+    for c in candidates():
+        path = os.path.join(TUNE_DIR, "lib_%s.so" % variant_name(c))
+        if not os.path.exists(path):
+            raise FileNotFoundError(path + " (run `python -m pscf_amd.autotune build` first)")
+        lib = Lib(path)
+        for n in sizes:
+            t0 = time.perf_counter()
+            us, chk = time_sweeps(lib, n, reps)
+            res.setdefault(str(n), {})[variant_name(c)] = us
+            print("%-10s n=%-3d %8.2f us/step  (checksum %.12e, %.1fs)" %
+                  (variant_name(c), n, us, chk, time.perf_counter() - t0), flush=True)
+        json.dump(res, open(results, "w"), indent=1, sort_keys=True)
+    return res
+
+
+def write(results=RESULTS):
+    res = json.load(open(results))
+    rows = []
+    for n in sorted(res, key=int):
+        best = min(res[n], key=res[n].get)
+        r, p, z = [int(t[1:]) for t in best.split("_")]
+        rows.append((int(n), r, p, z, res[n][best], res[n].get("r8_p0_z1")))
+    lines = [
+        "// Kernel plans per FFT/grid length for gfx950, written by pscf_amd/autotune.py",
+        "// from timings of the contour sweeps on an MI355X (rerun it to regenerate).",
+        "//   max_radix  largest Stockham radix of a transform of this length (8 or 4)",
+        "//   xpad       LDS padding (complex words) per packed row of an XY plane",
+        "//   zsplit     workgroups sharing one ky row of the z-column step kernel",
+        "#pragma once", "", "namespace pscf {", "",
+        "struct TunedPlan {", "  int n, max_radix, xpad, zsplit;", "};", "",
+        "constexpr TunedPlan kTunedPlans[] = {"]
+    for n, r, p, z, us, base in rows:
+        note = "  // %.2f us/step (radix 8, no pad, 1 row: %s)" % (
+            us, "%.2f" % base if base is not None else "n/a")
+        lines.append("    {%d, %d, %d, %d},%s" % (n, r, p, z, note))
+    lines += ["};", "", "}  // namespace pscf", ""]
+    open(TUNED_HPP, "w").write("\n".join(lines))
+    print("wrote", TUNED_HPP)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m pscf_amd.autotune")
+    ap.add_argument("cmd", choices=["build", "measure", "write"])
+    ap.add_argument("--sizes", default="32,48,64,96")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--results", default=RESULTS, help="timing table (json)")
+    a = ap.parse_args(argv)
+    if a.cmd == "build":
+        build(a.jobs)
+    elif a.cmd == "measure":
+        measure([int(v) for v in a.sizes.split(",")], a.reps, a.results)
+    else:
+        write(a.results)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

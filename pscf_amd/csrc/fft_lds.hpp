@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "tuned.hpp"
+
 namespace pscf {
 
 // ---------------------------------------------------------------- complex ops
@@ -29,21 +31,40 @@ template <int S> __device__ __forceinline__ double2 cmul_si(double2 a) {
 }
 
 // ------------------------------------------------------ compile-time factoring
-// Build-time tuning knobs (the variants an autotuning build sweeps):
-//   PSCF_MAX_RADIX  largest radix of a pass (8 or 4)
-//   PSCF_XPAD       complex words of padding per packed row of a plane; with
-//                   padding the x passes map consecutive threads to
-//                   consecutive rows, whose LDS addresses then fall in
-//                   different banks
-#ifndef PSCF_MAX_RADIX
-#define PSCF_MAX_RADIX 8
+// Per-length kernel plan: the tuned table (tuned.hpp) or, for lengths it does
+// not list, radix 8, no padding, one workgroup per z row.  PSCF_MAX_RADIX,
+// PSCF_XPAD and PSCF_ZSPLIT, when defined, override the plan for every length
+// (the variants the autotuner builds and times).
+constexpr TunedPlan plan_for(int n) {
+  for (const TunedPlan& p : kTunedPlans)
+    if (p.n == n) return p;
+  return TunedPlan{n, 8, 0, 1};
+}
+constexpr int plan_max_radix(int n) {
+#ifdef PSCF_MAX_RADIX
+  return PSCF_MAX_RADIX;
+#else
+  return plan_for(n).max_radix;
 #endif
-#ifndef PSCF_XPAD
-#define PSCF_XPAD 0
+}
+constexpr int plan_xpad(int n) {
+#ifdef PSCF_XPAD
+  return PSCF_XPAD;
+#else
+  return plan_for(n).xpad;
 #endif
+}
+constexpr int plan_zsplit(int n) {
+#ifdef PSCF_ZSPLIT
+  return PSCF_ZSPLIT;
+#else
+  return plan_for(n).zsplit;
+#endif
+}
 constexpr int fft_nfac(int n) {
   int c = 0;
-  while (PSCF_MAX_RADIX >= 8 && n > 1 && n % 8 == 0) { n /= 8; ++c; }
+  const bool r8 = plan_max_radix(n) >= 8;
+  while (r8 && n > 1 && n % 8 == 0) { n /= 8; ++c; }
   while (n % 4 == 0) { n /= 4; ++c; }
   while (n % 2 == 0) { n /= 2; ++c; }
   while (n % 3 == 0) { n /= 3; ++c; }
@@ -52,7 +73,8 @@ constexpr int fft_nfac(int n) {
 }
 constexpr int fft_fac(int n, int p) {
   int c = 0;
-  while (PSCF_MAX_RADIX >= 8 && n > 1 && n % 8 == 0) { if (c == p) return 8; n /= 8; ++c; }
+  const bool r8 = plan_max_radix(n) >= 8;
+  while (r8 && n > 1 && n % 8 == 0) { if (c == p) return 8; n /= 8; ++c; }
   while (n % 4 == 0) { if (c == p) return 4; n /= 4; ++c; }
   while (n % 2 == 0) { if (c == p) return 2; n /= 2; ++c; }
   while (n % 3 == 0) { if (c == p) return 3; n /= 3; ++c; }
@@ -213,10 +235,10 @@ __device__ __forceinline__ void fft_lines(double2* buf, const double2* __restric
 template <int N1, int N2> struct Plane {
   static constexpr int N1H = N1 / 2 + 1;
   static constexpr int NP = (N2 + 1) / 2;        // packed row pairs
-  static constexpr int PLS = N1 + PSCF_XPAD;     // LDS stride of a packed row
+  static constexpr int PLS = N1 + plan_xpad(N1);  // LDS stride of a packed row
   static constexpr int PSIZE = PLS * NP;         // complex words in P
   static constexpr int HSIZE = N1H * N2;         // complex words in H
-  static constexpr bool XJFAST = PSCF_XPAD == 0; // thread mapping of the x passes
+  static constexpr bool XJFAST = plan_xpad(N1) == 0;  // thread mapping of the x passes
   static_assert(N1 % 2 == 0, "first grid dimension must be even");
   // LDS index of element x of packed row pl, given i = pl*N1 + x
   static __device__ __forceinline__ int pidx(int i) { return (i / N1) * PLS + i % N1; }

@@ -344,20 +344,28 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __r
 }
 
 // ----------------------------------------------------------------------------
-// Z-column convolution (one workgroup per ky row, sweep and job): forward z
-// transform, multiply by the reciprocal-space factor, inverse z transform.
-// Jobs: kind 0 -> (A, k1) and (B, k2); kind 1 -> (C, k2).
+// Z-column convolution (one workgroup per ky row part, sweep and job): forward
+// z transform, multiply by the reciprocal-space factor, inverse z transform.
+// Jobs: kind 0 -> (A, k1) and (B, k2); kind 1 -> (C, k2).  A ky row's N1/2+1
+// kx columns are split over plan_zsplit(N1) workgroups (blockIdx.x = ky *
+// zsplit + part) of zrow_width columns each, the last one possibly narrower.
 // ----------------------------------------------------------------------------
+constexpr int zrow_width(int n1) { return (n1 / 2 + 1 + plan_zsplit(n1) - 1) / plan_zsplit(n1); }
+
 template <int N1, int N3>
-__global__ void __launch_bounds__(zcol_nt(N1 / 2 + 1, N3)) k_step_z(const StepTask* __restrict__ tasks, int j, int N2,
+__global__ void __launch_bounds__(zcol_nt(zrow_width(N1), N3)) k_step_z(const StepTask* __restrict__ tasks, int j, int N2,
                                                int kind, const double2* __restrict__ tw3g) {
-  constexpr int NT = zcol_nt(N1 / 2 + 1, N3);
+  constexpr int KW = zrow_width(N1);
+  constexpr int NT = zcol_nt(KW, N3);
   constexpr int N1H = N1 / 2 + 1;
-  constexpr int TOT = N1H * N3;
+  constexpr int TOT = KW * N3;
   constexpr int PER = (TOT + NT - 1) / NT;
   __shared__ double2 T[TOT];
   __shared__ double2 TW3[N3];
-  const int tid = threadIdx.x, ky = blockIdx.x, job = blockIdx.z;
+  constexpr int ZS = plan_zsplit(N1);
+  const int tid = threadIdx.x, ky = blockIdx.x / ZS, job = blockIdx.z;
+  const int kx0 = (blockIdx.x % ZS) * KW;
+  const int kw = N1H - kx0 < KW ? N1H - kx0 : KW;     // columns of this part
   const StepTask& t = tasks[blockIdx.y];
   if (t.skip || j >= t.ns) return;
   const int b = task_block(t, j);
@@ -371,34 +379,39 @@ __global__ void __launch_bounds__(zcol_nt(N1 / 2 + 1, N3)) k_step_z(const StepTa
     kf = t.k2[b];
   }
   const long rowstride = (long)N1H * N2;
+  const long rbase = (long)ky * N1H + kx0;
   double f[PER];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int i = tid + p * NT;
+    f[p] = 0.0;
     if (i < TOT) {
-      const long g = (i / N1H) * rowstride + (long)ky * N1H + i % N1H;
-      T[i] = arr[g];
-      f[p] = kf[g];
+      T[i] = make_double2(0.0, 0.0);
+      if (i % KW < kw) {
+        const long g = (i / KW) * rowstride + rbase + i % KW;
+        T[i] = arr[g];
+        f[p] = kf[g];
+      }
     }
   }
   stage<N3, NT>(TW3, tw3g, tid);
   __syncthreads();
-  fft_lines<N3, -1, N1H, 1, N1H, false, NT>(T, TW3, tid);
+  fft_lines<N3, -1, KW, 1, KW, false, NT>(T, TW3, tid);
   double2* keep = t.keepK[kind == 0 ? job : 2];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int i = tid + p * NT;
     if (i < TOT) {
-      if (keep) keep[(i / N1H) * rowstride + (long)ky * N1H + i % N1H] = T[i];
+      if (keep && i % KW < kw) keep[(i / KW) * rowstride + rbase + i % KW] = T[i];
       T[i] = cscale(T[i], f[p]);
     }
   }
   __syncthreads();
-  fft_lines<N3, +1, N1H, 1, N1H, false, NT>(T, TW3, tid);
+  fft_lines<N3, +1, KW, 1, KW, false, NT>(T, TW3, tid);
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int i = tid + p * NT;
-    if (i < TOT) arr[(i / N1H) * rowstride + (long)ky * N1H + i % N1H] = T[i];
+    if (i < TOT && i % KW < kw) arr[(i / KW) * rowstride + rbase + i % KW] = T[i];
   }
 }
 
@@ -409,26 +422,31 @@ __global__ void __launch_bounds__(zcol_nt(N1 / 2 + 1, N3)) k_step_z(const StepTa
 // pre-factor fields of the step on q (suffix q) and on a (suffix a):
 //   d<a, T q>/d omega = -(ds/3) [ e2 (a Wq + q Wa) + 2 e1 Va Vq - (e1/2)(a Uq + q Ua) ]
 // for T = (4 e2 C2 e1 C2 e2 - e1 C1 e1)/3 (src/step_mod.f90:156-174),
-// accumulated into the omega gradient of the step's monomer.
+// accumulated into the omega gradient of the step's block.  The pairs of a
+// chain share their block accumulators, so every thread applies all pairs to
+// its points in turn (no two threads update one point).
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ tasks,
-                                                  const int2* __restrict__ pairs, int j, long m) {
-  const StepTask& A = tasks[pairs[blockIdx.y].x];
-  const StepTask& Q = tasks[pairs[blockIdx.y].y];
-  if (A.skip || j < 0 || j >= A.ns) return;
-  const int b = task_block(A, j);
-  const double* a = A.slab + (long)task_in(A, j) * m;
-  const double* q = Q.slab + (long)task_in(Q, j) * m;
-  const double* e1 = A.e1[b];
-  const double* e2 = A.e2[b];
-  const double c = A.ds[b] / 3.0;
-  double* grad = A.grad[b];
-  for (long r = blockIdx.x * 256L + threadIdx.x; r < m; r += (long)gridDim.x * 256L) {
-    const double av = a[r], qv = q[r];
-    const double g = e2[r] * (av * Q.keepW[r] + qv * A.keepW[r]) +
-                     2.0 * e1[r] * A.keepV[r] * Q.keepV[r] -
-                     0.5 * e1[r] * (av * Q.keepU[r] + qv * A.keepU[r]);
-    grad[r] -= c * g;
+                                                  const int2* __restrict__ pairs, int npair, int j,
+                                                  long m) {
+  for (int pr = 0; pr < npair; ++pr) {
+    const StepTask& A = tasks[pairs[pr].x];
+    const StepTask& Q = tasks[pairs[pr].y];
+    if (A.skip || j < 0 || j >= A.ns) continue;
+    const int b = task_block(A, j);
+    const double* a = A.slab + (long)task_in(A, j) * m;
+    const double* q = Q.slab + (long)task_in(Q, j) * m;
+    const double* e1 = A.e1[b];
+    const double* e2 = A.e2[b];
+    const double c = A.ds[b] / 3.0;
+    double* grad = A.grad[b];
+    for (long r = blockIdx.x * 256L + threadIdx.x; r < m; r += (long)gridDim.x * 256L) {
+      const double av = a[r], qv = q[r];
+      const double g = e2[r] * (av * Q.keepW[r] + qv * A.keepW[r]) +
+                       2.0 * e1[r] * A.keepV[r] * Q.keepV[r] -
+                       0.5 * e1[r] * (av * Q.keepU[r] + qv * A.keepU[r]);
+      grad[r] -= c * g;
+    }
   }
 }
 
@@ -441,26 +459,28 @@ __global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ t
 // A half-grid point with 0 < kx < N1/2 stands for k and -k and counts twice.
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_kgrad(const StepTask* __restrict__ tasks,
-                                                   const int2* __restrict__ pairs, int j, int n1,
-                                                   long mk, double rM) {
-  const StepTask& A = tasks[pairs[blockIdx.y].x];
-  const StepTask& Q = tasks[pairs[blockIdx.y].y];
-  if (A.skip || j < 0 || j >= A.ns) return;
-  const int b = task_block(A, j);
-  const double* k1 = A.k1[b];
-  const double* k2 = A.k2[b];
-  const double c = A.ds[b] * A.bsq[b] / rM;
+                                                   const int2* __restrict__ pairs, int npair, int j,
+                                                   int n1, long mk, double rM) {
   const int n1h = n1 / 2 + 1;
-  double* kg = A.kgrad[b];
-  for (long g = blockIdx.x * 256L + threadIdx.x; g < mk; g += (long)gridDim.x * 256L) {
-    const int kx = (int)(g % n1h);
-    const double mult = (kx == 0 || 2 * kx == n1) ? 1.0 : 2.0;
-    const double2 aq = Q.keepK[0][g], aa = A.keepK[0][g];
-    const double2 bq = Q.keepK[1][g], ba = A.keepK[1][g];
-    const double2 cq = Q.keepK[2][g], ca = A.keepK[2][g];
-    const double r1 = aq.x * aa.x + aq.y * aa.y;
-    const double r2 = cq.x * ba.x + cq.y * ba.y + bq.x * ca.x + bq.y * ca.y;
-    kg[g] += mult * c * (k1[g] * r1 / 18.0 - k2[g] * r2 / 9.0);
+  for (int pr = 0; pr < npair; ++pr) {
+    const StepTask& A = tasks[pairs[pr].x];
+    const StepTask& Q = tasks[pairs[pr].y];
+    if (A.skip || j < 0 || j >= A.ns) continue;
+    const int b = task_block(A, j);
+    const double* k1 = A.k1[b];
+    const double* k2 = A.k2[b];
+    const double c = A.ds[b] * A.bsq[b] / rM;
+    double* kg = A.kgrad[b];
+    for (long g = blockIdx.x * 256L + threadIdx.x; g < mk; g += (long)gridDim.x * 256L) {
+      const int kx = (int)(g % n1h);
+      const double mult = (kx == 0 || 2 * kx == n1) ? 1.0 : 2.0;
+      const double2 aq = Q.keepK[0][g], aa = A.keepK[0][g];
+      const double2 bq = Q.keepK[1][g], ba = A.keepK[1][g];
+      const double2 cq = Q.keepK[2][g], ca = A.keepK[2][g];
+      const double r1 = aq.x * aa.x + aq.y * aa.y;
+      const double r2 = cq.x * ba.x + cq.y * ba.y + bq.x * ca.x + bq.y * ca.y;
+      kg[g] += mult * c * (k1[g] * r1 / 18.0 - k2[g] * r2 / 9.0);
+    }
   }
 }
 

@@ -121,6 +121,8 @@ struct PlaneSet {
 };
 struct ZSet {
   int nt_half = 0, nt_full = 0;   // workgroup sizes (zcol_nt for N1/2+1 and N1 wide rows)
+  int nt_step = 0;                // workgroup size of the step z kernel (zrow_width columns)
+  int zsplit = 1;                 // its workgroups per ky row
   StepZFn z = nullptr;
   ZfftFn half_f = nullptr, half_i = nullptr, full_f = nullptr, full_i = nullptr;
 };
@@ -143,6 +145,8 @@ ZSet make_z() {
   ZSet s;
   s.nt_half = zcol_nt(A / 2 + 1, C);
   s.nt_full = zcol_nt(A, C);
+  s.nt_step = zcol_nt(zrow_width(A), C);
+  s.zsplit = plan_zsplit(A);
   s.z = k_step_z<A, C>;
   s.half_f = k_zfft<A / 2 + 1, C, -1>;
   s.half_i = k_zfft<A / 2 + 1, C, +1>;
@@ -415,13 +419,13 @@ void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, h
                        g.tw1, g.tw2);
     if (j == max_ns) break;
     mark();
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 2), dim3(g.zs->nt_half), 0, s, d_tasks, j, g.n2,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, ntask, 2), dim3(g.zs->nt_step), 0, s, d_tasks, j, g.n2,
                        0, g.tw3);
     mark();
     hipLaunchKernelGGL(g.pl->c, dim3(g.n3, ntask, 2), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
                        g.tw1, g.tw2);
     mark();
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, ntask, 1), dim3(g.zs->nt_half), 0, s, d_tasks, j, g.n2,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, ntask, 1), dim3(g.zs->nt_step), 0, s, d_tasks, j, g.n2,
                        1, g.tw3);
   }
   mark();
@@ -898,17 +902,17 @@ void run_adj_sweeps(const System& S, hipStream_t s) {
     hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
                        g.tw2);
     if (j >= 1)
-      hipLaunchKernelGGL(k_adj_grad, dim3(gb, A.npair), dim3(256), 0, s, A.tasks.p, A.pairs.p, j - 1,
-                         g.m);
+      hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, A.tasks.p, A.pairs.p, A.npair,
+                         j - 1, g.m);
     if (j == max_ns) break;
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, nt, 2), dim3(g.zs->nt_half), 0, s, A.tasks.p, j, g.n2, 0,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 2), dim3(g.zs->nt_step), 0, s, A.tasks.p, j, g.n2, 0,
                        g.tw3);
     hipLaunchKernelGGL(g.pl->c, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
                        g.tw2);
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2, nt, 1), dim3(g.zs->nt_half), 0, s, A.tasks.p, j, g.n2, 1,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 1), dim3(g.zs->nt_step), 0, s, A.tasks.p, j, g.n2, 1,
                        g.tw3);
     if (A.with_k)
-      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk, A.npair), dim3(256), 0, s, A.tasks.p, A.pairs.p, j,
+      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk), dim3(256), 0, s, A.tasks.p, A.pairs.p, A.npair, j,
                          g.n1, g.mk, (double)g.m);
   }
   HIPCHK(hipGetLastError());

@@ -6,7 +6,8 @@ forward, and against the reference's own identities:
 * d/d(cell) [-(phi/N) ln Q] = scf_stress             (src/scf_mod.fp.f90:451-550)
 
 The adjoint differentiates the discrete map exactly, so finite differences
-agree to rounding and truncation error (tolerance 1e-6 relative).  The
+agree to rounding and truncation error (tolerance 1e-6 relative, fourth-order
+central differences).  The
 identities hold for the continuous contour only; their gap must shrink at
 the contour discretisation's order as ds is halved.
 """
@@ -53,7 +54,13 @@ def _make(name, backend, step=None):
 
 
 def _fd(f, h):
-    return (f(h) - f(-h)) / (2.0 * h)
+    """Fourth-order central difference: truncation O(h^4) and rounding
+    O(eps |f| / h) both stay far below the 1e-6 tolerance at h ~ 1e-3, where
+    the two-point form at h = 1e-6 is already at its rounding floor (~1e-7)."""
+    return (8.0 * (f(h) - f(-h)) - (f(2.0 * h) - f(-2.0 * h))) / (12.0 * h)
+
+
+H = 1e-3
 
 
 # contour steps chosen so that no block length sits on a rounding boundary of
@@ -80,14 +87,14 @@ def test_gradients_match_central_fd(backend, name, step):
 
     d = torch.tensor(rng.normal(size=omg.shape), device=dev)   # arbitrary, not symmetric
     with torch.no_grad():
-        fd = _fd(lambda h: loss(omg + h * d, cp0, L0).item(), 1e-6)
+        fd = _fd(lambda h: loss(omg + h * d, cp0, L0).item(), H)
     ad = (om.grad * d).sum().item()
     assert abs(ad - fd) <= FD_TOL * abs(fd), ("omega", ad, fd)
 
     for p in range(len(cp0)):
         e = torch.zeros_like(cp0)
         e[p] = 1.0
-        h = 1e-6 * cp0[p].item()
+        h = H * cp0[p].item()
         with torch.no_grad():
             fd = _fd(lambda t: loss(omg, cp0 + t * e, L0).item(), h)
         assert abs(cp.grad[p].item() - fd) <= FD_TOL * abs(fd), ("cell", p, cp.grad[p].item(), fd)
@@ -95,7 +102,7 @@ def test_gradients_match_central_fd(backend, name, step):
     for b in range(len(L0)):
         e = torch.zeros_like(L0)
         e[b] = 1.0
-        h = 1e-6 * L0[b].item()
+        h = H * L0[b].item()
         with torch.no_grad():
             fd = _fd(lambda t: loss(omg, cp0, L0 + t * e).item(), h)
         assert abs(Lb.grad[b].item() - fd) <= FD_TOL * max(abs(fd), 1e-3), ("length", b,
@@ -114,7 +121,7 @@ def test_free_energy_gradient_matches_fd(backend):
     F(om).backward()
     d = torch.tensor(rng.normal(size=omg.shape), device=dev)
     with torch.no_grad():
-        fd = _fd(lambda h: F(omg + h * d).item(), 1e-6)
+        fd = _fd(lambda h: F(omg + h * d).item(), H)
     ad = (om.grad * d).sum().item()
     assert abs(ad - fd) <= FD_TOL * abs(fd), (ad, fd)
 
@@ -141,7 +148,7 @@ def test_star_free_energy_value_and_gradient(backend):
     for _ in range(2):
         d = torch.tensor(rng.normal(size=w0.shape), device=dev)
         with torch.no_grad():
-            fd = _fd(lambda h: F(w0 + h * d).item(), 1e-6)
+            fd = _fd(lambda h: F(w0 + h * d).item(), H)
         ad = (w.grad * d).sum().item()
         assert abs(ad - fd) <= FD_TOL * abs(fd), (ad, fd)
 
@@ -222,10 +229,13 @@ def test_gyroid64_loss_gradients_gpu():
     loss(om, cp, fA).backward()
     d = torch.tensor(rng.normal(size=omg.shape), device=dev)
     with torch.no_grad():
-        fd_w = _fd(lambda h: loss(omg + h * d, cp0, f0).item(), 1e-6)
-        fd_a = _fd(lambda h: loss(omg, cp0 + h, f0).item(), 16e-6)
-        fd_f = _fd(lambda h: loss(omg, cp0, f0 + h).item(), 1e-6)
+        fd_w = _fd(lambda h: loss(omg + h * d, cp0, f0).item(), H)
+        fd_a = _fd(lambda h: loss(omg, cp0 + h, f0).item(), 16.0 * H)
+        fd_f = _fd(lambda h: loss(omg, cp0, f0 + h).item(), H)
+        # the two-point difference at step 1e-6, to its own rounding floor
+        fd2_w = (loss(omg + 1e-6 * d, cp0, f0).item() - loss(omg - 1e-6 * d, cp0, f0).item()) / 2e-6
     ad_w = (om.grad * d).sum().item()
     assert abs(ad_w - fd_w) <= FD_TOL * abs(fd_w), (ad_w, fd_w)
     assert abs(cp.grad.item() - fd_a) <= FD_TOL * abs(fd_a), (cp.grad.item(), fd_a)
     assert abs(fA.grad.item() - fd_f) <= FD_TOL * abs(fd_f), (fA.grad.item(), fd_f)
+    assert abs(ad_w - fd2_w) <= 1e-5 * abs(fd2_w), (ad_w, fd2_w)
