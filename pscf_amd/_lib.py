@@ -99,6 +99,13 @@ class Lib:
     tensor_device = "cuda"
 
     def __init__(self, path=None):
+        # PyTorch ships its own HIP runtime: when torch is present it must be
+        # loaded first, so that this library binds to the same runtime
+        # (libamdhip64 loaded by this library first leaves torch without devices)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         path = path or DEFAULT_LIB
         if not os.path.exists(path):
             raise PscfError("libpscf_hip.so not built at %s: run `python -c 'import "

@@ -48,7 +48,8 @@ struct StepTask {
   int skip;                     // 1: inactive (converged instance of a batch): every kernel returns
   int rev;                      // 1: step j is the step j' = ns-1-j (adjoint recomputation)
   int reload;                   // 1: every step reads its input from the slab (no carry)
-  int add_src;                  // 1: out = T in + src[out] (adjoint sweep with sources)
+  int add_src;                  // 1: adjoint sweep, out = T in + g(out), g computed in place
+  int src_end;                  // contour point that also receives scal[1] (-1: none)
   int nostore;                  // 1: step outputs are not written to the slab
   int nblk;
   int bnd[MAXBLK + 1];
@@ -60,7 +61,12 @@ struct StepTask {
   double2* B;                   // first half-step spectrum  (Mk)
   double2* C;                   // second half-step spectrum (Mk)
   double* q1;                   // full-step result          (M)
-  const double* src;            // source slab (add_src; its first point is also the start value)
+  // adjoint sources g(s) = scal[0] sum_b (ds_b/3) w_b(s) lam_b oth(s) [+ scal[1] at src_end],
+  // w_b: Simpson weights of block b (1 at its ends, 4 odd, 2 even interior points)
+  const double* oth;            // the other sweep's q slab
+  const double* lam[MAXBLK];    // dL/drho of each block's monomer (M)
+  const double* scal;           // coefficient, end term
+  double* keepA;                // step inputs, two buffers by step parity (2 M)
   // adjoint support: pre-factor fields of the step kept for the gradient,
   // U = C1[e1 q], V = C2[e2 q], W = C2[e1 V] (C: normalised convolutions)
   double* keepU;
@@ -88,6 +94,19 @@ __device__ __forceinline__ int task_block(const StepTask& t, int j) {
   int b = 0;
   while (b + 1 < t.nblk && lo >= t.bnd[b + 1]) ++b;
   return b;
+}
+
+__device__ __forceinline__ double adj_source(const StepTask& t, int sp, long idx, long M) {
+  double wsum = 0.0;
+  for (int b = 0; b < t.nblk; ++b) {
+    const int lo = t.bnd[b], hi = t.bnd[b + 1];
+    if (sp < lo || sp > hi) continue;
+    const double w = (sp == lo || sp == hi) ? 1.0 : (((sp - lo) & 1) ? 4.0 : 2.0);
+    wsum += t.ds[b] / 3.0 * w * t.lam[b][idx];
+  }
+  double v = t.scal[0] * wsum * t.oth[(long)sp * M + idx];
+  if (sp == t.src_end) v += t.scal[1];
+  return v;
 }
 
 // Copy n complex words global -> LDS (no barrier).
@@ -171,8 +190,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
     }
   }
   if (j == 0) {
-    double* qin = t.slab + (long)task_in(t, 0) * M + pbase;
-    const double* s0 = t.add_src ? t.src + (long)task_in(t, 0) * M + pbase : nullptr;
+    const int s_in = task_in(t, 0);
+    double* qin = t.add_src ? nullptr : t.slab + (long)s_in * M + pbase;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int i = tid + p * NT;
@@ -186,13 +205,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
             qin[o0] = 1.0;
             if (two) qin[o0 + N1] = 1.0;
           }
-        } else if (s0) {
-          qa[p] = s0[o0];
-          qb[p] = two ? s0[o0 + N1] : 0.0;
-          if (job == 0) {
-            qin[o0] = qa[p];
-            if (two) qin[o0 + N1] = qb[p];
-          }
+        } else if (t.add_src) {
+          qa[p] = adj_source(t, s_in, pbase + o0, M);
+          qb[p] = two ? adj_source(t, s_in, pbase + o0 + N1, M) : 0.0;
         } else {
           qa[p] = qin[o0];
           qb[p] = two ? qin[o0 + N1] : 0.0;
@@ -203,8 +218,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
     stage<PL::HSIZE, NT>(H, t.C + hbase, tid);
     __syncthreads();
     plane_inverse<N1, N2, NT>(H, P, TW1, TW2, tid);
-    double* qo = t.slab + (long)task_out(t, j - 1) * M + pbase;
-    const double* so = t.add_src ? t.src + (long)task_out(t, j - 1) * M + pbase : nullptr;
+    const int s_out = task_out(t, j - 1);
+    double* qo = t.nostore ? nullptr : t.slab + (long)s_out * M + pbase;
     double* kw = t.keepW ? t.keepW + pbase : nullptr;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -220,9 +235,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
           const double q2b = e2b[p] * r.y / rM;
           vb = (4.0 * q2b - q1b[p]) / 3.0;
         }
-        if (so) {
-          va += so[o0];
-          if (two) vb += so[o0 + N1];
+        if (t.add_src) {
+          va += adj_source(t, s_out, pbase + o0, M);
+          if (two) vb += adj_source(t, s_out, pbase + o0 + N1, M);
         }
         if (!t.reload) {
           qa[p] = va;
@@ -243,10 +258,18 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
     __syncthreads();
   }
   if (!start) return;
+  double* ka = (t.keepA && job == 0) ? t.keepA + (j & 1) * M + pbase : nullptr;
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int i = tid + p * NT;
-    if (i < NPX) P[PL::pidx(i)] = make_double2(fa[p] * qa[p], fb[p] * qb[p]);
+    if (i < NPX) {
+      P[PL::pidx(i)] = make_double2(fa[p] * qa[p], fb[p] * qb[p]);
+      if (ka) {
+        const int o0 = 2 * (i / N1) * N1 + i % N1;
+        ka[o0] = qa[p];
+        if (2 * (i / N1) + 1 < N2) ka[o0 + N1] = qb[p];
+      }
+    }
   }
   __syncthreads();
   plane_forward<N1, N2, NT>(P, H, TW1, TW2, tid);
@@ -434,7 +457,7 @@ __global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ t
     const StepTask& Q = tasks[pairs[pr].y];
     if (A.skip || j < 0 || j >= A.ns) continue;
     const int b = task_block(A, j);
-    const double* a = A.slab + (long)task_in(A, j) * m;
+    const double* a = A.keepA + (j & 1) * m;    // the adjoint entering step j (K_A(j) kept it)
     const double* q = Q.slab + (long)task_in(Q, j) * m;
     const double* e1 = A.e1[b];
     const double* e2 = A.e2[b];

@@ -302,47 +302,6 @@ __global__ void __launch_bounds__(NTA) k_partial_qlogq(const double* __restrict_
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// ----------------------------------------------------------------------------
-// Adjoint of density() with respect to the omega grid fields.
-//
-// Sources (derivative of L(phi, ln Q) with respect to each stored q):
-//   g_f(s) = c sum_{b containing s} (ds_b/3) w_b(s) lam_{mon b} q_r(s)  [+ Gam/(Q M) at s = ns]
-//   g_r(s) = c sum_{b containing s} (ds_b/3) w_b(s) lam_{mon b} q_f(s)
-// with c = phi/(N Q), Simpson weights w = 1 at block ends, 4 at odd and 2 at
-// even interior offsets (src/scf_mod.fp.f90:362-424), junction points in both
-// blocks; written over the whole adjoint slabs af, ar ((ns+1) M each).
-// ----------------------------------------------------------------------------
-struct AdjSrcArgs {
-  int nblk, ns;
-  int bnd[17];
-  double ds[16];
-  int monomer[16];
-  double coef;          // phi / (N Q)
-  double endterm;       // Gam / (Q M), added to g_f(ns)
-};
-
-__global__ void __launch_bounds__(NTA) k_adj_sources(const double* __restrict__ qf,
-                                                     const double* __restrict__ qr, long m,
-                                                     AdjSrcArgs a, const double* __restrict__ lam,
-                                                     double* __restrict__ af,
-                                                     double* __restrict__ ar) {
-  const long n = (long)(a.ns + 1) * m;
-  for (long gi = blockIdx.x * (long)NTA + threadIdx.x; gi < n; gi += (long)gridDim.x * NTA) {
-    const int s = (int)(gi / m);
-    const long r = gi % m;
-    double wsum = 0.0;
-    for (int b = 0; b < a.nblk; ++b) {
-      const int lo = a.bnd[b], hi = a.bnd[b + 1];
-      if (s < lo || s > hi) continue;
-      double w = 1.0;
-      if (s != lo && s != hi) w = ((s - lo) % 2 == 1) ? 4.0 : 2.0;
-      wsum += a.ds[b] / 3.0 * w * lam[(long)a.monomer[b] * m + r];
-    }
-    const double c = a.coef * wsum;
-    af[gi] = c * qr[gi] + (s == a.ns ? a.endterm : 0.0);
-    ar[gi] = c * qf[gi];
-  }
-}
 
 // ----------------------------------------------------------------------------
 // Adjoint support.

@@ -289,9 +289,9 @@ struct ChainDev {
 struct AdjState {
   int ntask = 0, npair = 0;
   bool with_k = false;
-  DevBuf<double> aslab;    // per chain: a_f, a_r slabs ((ns+1) m each)
-  DevBuf<double> gslab;    // per chain: sources g_f, g_r ((ns+1) m each)
-  DevBuf<double> keep;     // per task: U, V, W (m each)
+  DevBuf<double> keep;     // per task: U, V, W (m each); adjoint tasks also 2 m of kept inputs
+  DevBuf<double> lam;      // dL/drho_grid of the current call (m per monomer)
+  DevBuf<double> scal;     // per chain: source coefficient, end term
   DevBuf<double2> keepk;   // per task: 3 spectra (mk each), with_k only
   DevBuf<double2> abc;     // per task: A, B, C (mk each)
   DevBuf<double> q1;       // per task (m)
@@ -301,7 +301,7 @@ struct AdjState {
   DevBuf<int2> pairs;
   DevBuf<double> part;     // reduction partials, and one result slot at the end
   std::vector<StepTask> h_tasks;
-  std::vector<size_t> slab_off, blk_off;   // per chain: offset into aslab/gslab, into gblk/kblk (blocks)
+  std::vector<size_t> blk_off;   // per chain: first block in gblk/kblk
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~AdjState() {
@@ -815,16 +815,14 @@ void adjoint_setup(System& S, bool with_k) {
   A->with_k = with_k;
   A->ntask = 4 * S.nchain;
   A->npair = 2 * S.nchain;
-  size_t nslab = 0, nblk = 0;
+  size_t nblk = 0;
   for (int c = 0; c < S.nchain; ++c) {
-    A->slab_off.push_back(nslab);
     A->blk_off.push_back(nblk);
-    nslab += (size_t)2 * (S.chains[c]->ns + 1) * g.m;
     nblk += S.chains[c]->nblk;
   }
-  A->aslab.alloc(nslab);
-  A->gslab.alloc(nslab);
-  A->keep.alloc((size_t)3 * g.m * A->ntask);
+  A->keep.alloc((size_t)5 * g.m * A->ntask);
+  A->lam.alloc((size_t)S.nmon * g.m);
+  A->scal.alloc((size_t)2 * S.nchain);
   if (with_k) {
     A->keepk.alloc((size_t)3 * g.mk * A->ntask);
     A->kblk.alloc((size_t)g.mk * nblk);
@@ -839,27 +837,31 @@ void adjoint_setup(System& S, bool with_k) {
   std::vector<int2> hp(A->npair);
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
-    const size_t sl = (size_t)(ch.ns + 1) * g.m;
-    double* af = A->aslab.p + A->slab_off[c];
-    double* ar = af + sl;
-    const double* gf = A->gslab.p + A->slab_off[c];
-    const double* gr = gf + sl;
     // task order per chain: a_f, q_f recomputed, a_r, q_r recomputed
     for (int k = 0; k < 4; ++k) {
       const int ti = 4 * c + k;
       StepTask t = S.h_tasks[2 * c + (k < 2 ? 0 : 1)];   // factors, blocks, ns of the chain
       t.init_one = 0;
       t.skip = 0;
-      if (k == 0 || k == 2) {       // adjoint sweeps, opposite direction to their q
+      t.keepA = nullptr;
+      t.oth = nullptr;
+      t.scal = nullptr;
+      t.src_end = -1;
+      for (int b = 0; b < MAXBLK; ++b) t.lam[b] = nullptr;
+      if (k == 0 || k == 2) {       // adjoint sweeps, opposite direction to their q; no slab
         t.dir = k == 0 ? -1 : 1;
-        t.slab = k == 0 ? af : ar;
-        t.src = k == 0 ? gf : gr;
+        t.slab = nullptr;
         t.add_src = 1;
-        t.rev = t.reload = t.nostore = 0;
+        t.nostore = 1;
+        t.rev = t.reload = 0;
+        t.oth = k == 0 ? ch.qr.p : ch.qf.p;        // g_f uses q_r, g_r uses q_f
+        t.src_end = k == 0 ? ch.ns : -1;           // dL/dQ enters a_f at s = Ns
+        t.scal = A->scal.p + 2 * c;
+        for (int b = 0; b < ch.nblk; ++b) t.lam[b] = A->lam.p + (size_t)ch.mon[b] * g.m;
+        t.keepA = A->keep.p + ((size_t)5 * ti + 3) * g.m;
       } else {                      // recomputed forward sweeps
         t.dir = k == 1 ? 1 : -1;
         t.slab = k == 1 ? ch.qf.p : ch.qr.p;
-        t.src = nullptr;
         t.add_src = 0;
         t.rev = t.reload = t.nostore = 1;
       }
@@ -867,7 +869,7 @@ void adjoint_setup(System& S, bool with_k) {
       t.B = t.A + g.mk;
       t.C = t.B + g.mk;
       t.q1 = A->q1.p + (size_t)ti * g.m;
-      t.keepU = A->keep.p + (size_t)(3 * ti) * g.m;
+      t.keepU = A->keep.p + (size_t)(5 * ti) * g.m;
       t.keepV = t.keepU + g.m;
       t.keepW = t.keepV + g.m;
       for (int q = 0; q < 3; ++q)
@@ -972,25 +974,14 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
     for (int b = 0; b < ch.nblk; ++b) tsum += T[c][b];
     gam[c] = (dQ ? dQ[c] : 0.0) - (S.ensemble == 1 ? 0.0 : coef[c] * tsum / Q);
   }
-  // sources, zeroed accumulators
+  // sources: lam copied into the adjoint state, coefficient and end term per chain
+  HIPCHK(hipMemcpyAsync(A.lam.p, lam, sizeof(double) * S.nmon * g.m, hipMemcpyDeviceToDevice, s));
+  std::vector<double> hs(2 * S.nchain);
   for (int c = 0; c < S.nchain; ++c) {
-    ChainDev& ch = *S.chains[c];
-    AdjSrcArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.nblk = ch.nblk;
-    a.ns = ch.ns;
-    for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
-    for (int b = 0; b < ch.nblk; ++b) {
-      a.ds[b] = ch.ds[b];
-      a.monomer[b] = ch.mon[b];
-    }
-    a.coef = coef[c];
-    a.endterm = gam[c] / (double)g.m;
-    double* gf = A.gslab.p + A.slab_off[c];
-    double* gr = gf + (size_t)(ch.ns + 1) * g.m;
-    hipLaunchKernelGGL(k_adj_sources, dim3(grid_blocks((long)(ch.ns + 1) * g.m)), dim3(NTA), 0, s,
-                       ch.qf.p, ch.qr.p, g.m, a, lam, gf, gr);
+    hs[2 * c] = coef[c];
+    hs[2 * c + 1] = gam[c] / (double)g.m;
   }
+  HIPCHK(hipMemcpyAsync(A.scal.p, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, s));
   size_t nblk = 0;
   for (int c = 0; c < S.nchain; ++c) nblk += S.chains[c]->nblk;
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)(nblk * g.m))), dim3(NTA), 0, s, A.gblk.p,
