@@ -39,10 +39,17 @@ constexpr int c2c_nt(int n1, int n2) {
 // One contour sweep (a chain's q or q-dagger) advanced by the step kernels.
 // Step j maps contour index in(j) -> out(j): forward in = j, backward
 // in = ns - j.  The step interval [lo, lo+1] (lo = min(in, out)) belongs to
-// block b with bnd[b] <= lo < bnd[b+1].
+// block b with bnd[b] <= lo < bnd[b+1].  A task takes the steps
+// jbeg <= j < jend of its sweep, step j during the launch with index
+// j - jofs; its first step reads its input from `start` (if set) or from its
+// slab.  Contour point s lives at slab + slot(s)*M, slot(s) = pmap[s] (pmap
+// set; -1: the point is not stored) or s.
 struct StepTask {
-  double* slab;                 // (ns+1) * M, point s at slab + s*M
+  double* slab;                 // stored contour points
   int ns;
+  int jofs, jbeg, jend;         // launch offset and active step window
+  const int* pmap;              // contour point -> slot of slab (length ns+1), or null
+  const double* start;          // input of step jbeg (M), or null
   int dir;                      // +1 forward sweep, -1 backward sweep
   int init_one;                 // 1: the first point is the constant 1 (and is written)
   int skip;                     // 1: inactive (converged instance of a batch): every kernel returns
@@ -79,6 +86,7 @@ struct StepTask {
   double* kgrad[MAXBLK];        // |k|^2 gradient of the block on the half grid (Mk)
 };
 
+__device__ __forceinline__ int task_slot(const StepTask& t, int sp) { return t.pmap ? t.pmap[sp] : sp; }
 __device__ __forceinline__ int task_j(const StepTask& t, int j) { return t.rev ? t.ns - 1 - j : j; }
 __device__ __forceinline__ int task_in(const StepTask& t, int j) {
   j = task_j(t, j);
@@ -129,7 +137,7 @@ __device__ __forceinline__ void stage(double2* dst, const double2* __restrict__ 
 // twiddles are staged in LDS, so the transform passes wait on LDS only.
 // ----------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __restrict__ tasks, int j, int N3,
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __restrict__ tasks, int jl, int N3,
                                                const double2* __restrict__ tw1g,
                                                const double2* __restrict__ tw2g) {
   constexpr int NT = plane_nt(N1, N2);
@@ -141,12 +149,14 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   __shared__ double2 TW1[N1], TW2[N2];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
-  if (t.skip || j > t.ns || (job == 1 && j == t.ns)) return;
+  const int j = jl + t.jofs;
+  if (t.skip || j < t.jbeg || j > t.jend || (job == 1 && j == t.jend)) return;
   const long M = (long)N1 * N2 * N3;
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
   const long hbase = (long)z * PL::HSIZE;
-  const bool start = j < t.ns;
+  const bool start = j < t.jend;
+  const bool first = j == t.jbeg;
   const double* ef = start ? (job == 0 ? t.e1[task_block(t, j)] : t.e2[task_block(t, j)]) + pbase
                            : nullptr;
   double qa[PER], qb[PER], fa[PER], fb[PER], e2a[PER], e2b[PER], q1a[PER], q1b[PER];
@@ -162,7 +172,7 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
         fa[p] = ef[o0];
         if (two) fb[p] = ef[o0 + N1];
       }
-      if (j > 0) {
+      if (!first) {
         const double* e2 = t.e2[task_block(t, j - 1)] + pbase;
         const double* q1 = t.q1 + pbase;
         e2a[p] = e2[o0];
@@ -176,8 +186,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   }
   stage<N1, NT>(TW1, tw1g, tid);
   stage<N2, NT>(TW2, tw2g, tid);
-  if (t.reload && start) {
-    const double* qin = t.slab + (long)task_in(t, j) * M + pbase;
+  if (t.reload && start && !first) {
+    const double* qin = t.slab + (long)task_slot(t, task_in(t, j)) * M + pbase;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int i = tid + p * NT;
@@ -189,9 +199,11 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
       }
     }
   }
-  if (j == 0) {
-    const int s_in = task_in(t, 0);
-    double* qin = t.add_src ? nullptr : t.slab + (long)s_in * M + pbase;
+  if (first) {
+    const int s_in = task_in(t, j);
+    const int sl = t.slab ? task_slot(t, s_in) : -1;
+    double* qin = sl >= 0 ? t.slab + (long)sl * M + pbase : nullptr;
+    const double* st = t.start ? t.start + pbase : nullptr;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int i = tid + p * NT;
@@ -201,13 +213,20 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
         if (t.init_one) {
           qa[p] = 1.0;
           qb[p] = 1.0;
-          if (job == 0) {
+          if (job == 0 && qin && !t.nostore) {
             qin[o0] = 1.0;
             if (two) qin[o0 + N1] = 1.0;
           }
         } else if (t.add_src) {
           qa[p] = adj_source(t, s_in, pbase + o0, M);
           qb[p] = two ? adj_source(t, s_in, pbase + o0 + N1, M) : 0.0;
+        } else if (st) {
+          qa[p] = st[o0];
+          qb[p] = two ? st[o0 + N1] : 0.0;
+          if (job == 0 && qin && !t.nostore) {
+            qin[o0] = qa[p];
+            if (two) qin[o0 + N1] = qb[p];
+          }
         } else {
           qa[p] = qin[o0];
           qb[p] = two ? qin[o0 + N1] : 0.0;
@@ -219,7 +238,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
     __syncthreads();
     plane_inverse<N1, N2, NT>(H, P, TW1, TW2, tid);
     const int s_out = task_out(t, j - 1);
-    double* qo = t.nostore ? nullptr : t.slab + (long)s_out * M + pbase;
+    const int so = (t.nostore || !t.slab) ? -1 : task_slot(t, s_out);
+    double* qo = so >= 0 ? t.slab + (long)so * M + pbase : nullptr;
     double* kw = t.keepW ? t.keepW + pbase : nullptr;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -244,7 +264,7 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
           qb[p] = vb;
         }
         if (job == 0) {
-          if (!t.nostore) {
+          if (qo) {
             qo[o0] = va;
             if (two) qo[o0 + N1] = vb;
           }
@@ -288,7 +308,7 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
 // transform into C.  Reference: src/step_mod.f90:159-167.
 // ----------------------------------------------------------------------------
 template <int N1, int N2>
-__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __restrict__ tasks, int j, int N3,
+__global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __restrict__ tasks, int jl, int N3,
                                                const double2* __restrict__ tw1g,
                                                const double2* __restrict__ tw2g) {
   constexpr int NT = plane_nt(N1, N2);
@@ -300,7 +320,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __r
   __shared__ double2 TW1[N1], TW2[N2];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
-  if (t.skip || j >= t.ns) return;
+  const int j = jl + t.jofs;
+  if (t.skip || j < t.jbeg || j >= t.jend) return;
   const long M = (long)N1 * N2 * N3;
   const double rM = (double)M;
   const long pbase = (long)z * N1 * N2;
@@ -376,7 +397,7 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __r
 constexpr int zrow_width(int n1) { return (n1 / 2 + 1 + plan_zsplit(n1) - 1) / plan_zsplit(n1); }
 
 template <int N1, int N3>
-__global__ void __launch_bounds__(zcol_nt(zrow_width(N1), N3)) k_step_z(const StepTask* __restrict__ tasks, int j, int N2,
+__global__ void __launch_bounds__(zcol_nt(zrow_width(N1), N3)) k_step_z(const StepTask* __restrict__ tasks, int jl, int N2,
                                                int kind, const double2* __restrict__ tw3g) {
   constexpr int KW = zrow_width(N1);
   constexpr int NT = zcol_nt(KW, N3);
@@ -390,7 +411,8 @@ __global__ void __launch_bounds__(zcol_nt(zrow_width(N1), N3)) k_step_z(const St
   const int kx0 = (blockIdx.x % ZS) * KW;
   const int kw = N1H - kx0 < KW ? N1H - kx0 : KW;     // columns of this part
   const StepTask& t = tasks[blockIdx.y];
-  if (t.skip || j >= t.ns) return;
+  const int j = jl + t.jofs;
+  if (t.skip || j < t.jbeg || j >= t.jend) return;
   const int b = task_block(t, j);
   double2* arr;
   const double* kf;
@@ -450,15 +472,16 @@ __global__ void __launch_bounds__(zcol_nt(zrow_width(N1), N3)) k_step_z(const St
 // its points in turn (no two threads update one point).
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ tasks,
-                                                  const int2* __restrict__ pairs, int npair, int j,
+                                                  const int2* __restrict__ pairs, int npair, int jl,
                                                   long m) {
   for (int pr = 0; pr < npair; ++pr) {
     const StepTask& A = tasks[pairs[pr].x];
     const StepTask& Q = tasks[pairs[pr].y];
-    if (A.skip || j < 0 || j >= A.ns) continue;
+    const int j = jl + A.jofs, jq = jl + Q.jofs;
+    if (A.skip || j < A.jbeg || j >= A.jend) continue;
     const int b = task_block(A, j);
     const double* a = A.keepA + (j & 1) * m;    // the adjoint entering step j (K_A(j) kept it)
-    const double* q = Q.slab + (long)task_in(Q, j) * m;
+    const double* q = Q.slab + (long)task_slot(Q, task_in(Q, jq)) * m;
     const double* e1 = A.e1[b];
     const double* e2 = A.e2[b];
     const double c = A.ds[b] / 3.0;
@@ -482,13 +505,14 @@ __global__ void __launch_bounds__(256) k_adj_grad(const StepTask* __restrict__ t
 // A half-grid point with 0 < kx < N1/2 stands for k and -k and counts twice.
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_kgrad(const StepTask* __restrict__ tasks,
-                                                   const int2* __restrict__ pairs, int npair, int j,
+                                                   const int2* __restrict__ pairs, int npair, int jl,
                                                    int n1, long mk, double rM) {
   const int n1h = n1 / 2 + 1;
   for (int pr = 0; pr < npair; ++pr) {
     const StepTask& A = tasks[pairs[pr].x];
     const StepTask& Q = tasks[pairs[pr].y];
-    if (A.skip || j < 0 || j >= A.ns) continue;
+    const int j = jl + A.jofs;
+    if (A.skip || j < A.jbeg || j >= A.jend) continue;
     const int b = task_block(A, j);
     const double* k1 = A.k1[b];
     const double* k2 = A.k2[b];

@@ -605,6 +605,7 @@ void startup(System& S, double chain_step) {
       std::memset(&t, 0, sizeof t);
       t.slab = d == 0 ? ch.qf.p : ch.qr.p;
       t.ns = ch.ns;
+      t.jend = ch.ns;
       t.dir = d == 0 ? 1 : -1;
       t.init_one = 1;
       t.nblk = ch.nblk;
@@ -1537,6 +1538,7 @@ int pscf_step_exp(int64_t step, const double* q_in, double* q_out) {
     std::memset(&t, 0, sizeof t);
     t.slab = S.slab.p;
     t.ns = 1;
+    t.jend = 1;
     t.dir = 1;
     t.init_one = 0;
     t.nblk = 1;
