@@ -159,6 +159,14 @@ int pscf_system_get_grids(int64_t sys, double* omega_grid, double* rho_grid);
  * the captured sweep graph (slot 4 only); 2 eager launches with an event
  * around every kernel (all slots). */
 int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count);
+/* Checkpointed contour storage (no reference counterpart; the reference keeps
+ * all Ns+1 points of q and q-dagger, src/chain_mod.f90:160-163).  stride K > 0:
+ * density() keeps q, q-dagger only at the points s with s % K == 0 or
+ * (Ns - s) % K == 0 and recomputes the others segment by segment for the
+ * Simpson integrals and the adjoint; 0 restores full storage.  Re-runs
+ * startup when the system is started.  scf_stress and divide_energy need full
+ * storage and fail while it is on. */
+int pscf_system_set_checkpoint(int64_t sys, int stride);
 /* Capture the sweeps of density() in a HIP graph (default on). */
 int pscf_system_use_graph(int64_t sys, int enable);
 

@@ -70,7 +70,8 @@ struct StepTask {
   double* q1;                   // full-step result          (M)
   // adjoint sources g(s) = scal[0] sum_b (ds_b/3) w_b(s) lam_b oth(s) [+ scal[1] at src_end],
   // w_b: Simpson weights of block b (1 at its ends, 4 odd, 2 even interior points)
-  const double* oth;            // the other sweep's q slab
+  const double* oth;            // the other sweep's q points
+  const int* omap;              // their slots (as pmap), or null
   const double* lam[MAXBLK];    // dL/drho of each block's monomer (M)
   const double* scal;           // coefficient, end term
   double* keepA;                // step inputs, two buffers by step parity (2 M)
@@ -112,7 +113,7 @@ __device__ __forceinline__ double adj_source(const StepTask& t, int sp, long idx
     const double w = (sp == lo || sp == hi) ? 1.0 : (((sp - lo) & 1) ? 4.0 : 2.0);
     wsum += t.ds[b] / 3.0 * w * t.lam[b][idx];
   }
-  double v = t.scal[0] * wsum * t.oth[(long)sp * M + idx];
+  double v = t.scal[0] * wsum * t.oth[(long)(t.omap ? t.omap[sp] : sp) * M + idx];
   if (sp == t.src_end) v += t.scal[1];
   return v;
 }

@@ -329,6 +329,43 @@ __global__ void __launch_bounds__(NTA) k_simpson_lam(const double* __restrict__ 
   if (threadIdx.x == 0) part[(long)b * gridDim.x + blockIdx.x] = s;
 }
 
+// Checkpointed storage: Simpson integrals of one recomputed contour segment.
+// For the points s of [lo, hi) (and hi itself when `last`), held at
+// qf/qr + (s - lo)*M, every block b containing s gets
+//   blk[b*M + r] += ds_b/3 * w_b(s) * qf(s, r) * qr(s, r)
+// (w_b: 1 at the block's ends, 4 at odd and 2 at even interior offsets).
+__global__ void __launch_bounds__(NTA) k_simpson_seg(const double* __restrict__ qf,
+                                                     const double* __restrict__ qr, long m,
+                                                     SimpsonArgs a, int lo, int hi, int last,
+                                                     double* __restrict__ blk) {
+  const int s_end = last ? hi : hi - 1;
+  for (long r = blockIdx.x * (long)NTA + threadIdx.x; r < m; r += (long)gridDim.x * NTA) {
+    for (int sp = lo; sp <= s_end; ++sp) {
+      const double v = qf[(long)(sp - lo) * m + r] * qr[(long)(sp - lo) * m + r];
+      for (int b = 0; b < a.nblk; ++b) {
+        const int b0 = a.bnd[b], b1 = a.bnd[b + 1];
+        if (sp < b0 || sp > b1) continue;
+        const double w = (sp == b0 || sp == b1) ? 1.0 : (((sp - b0) & 1) ? 4.0 : 2.0);
+        blk[(long)b * m + r] += a.ds[b] / 3.0 * w * v;
+      }
+    }
+  }
+}
+
+// rho_grid(monomer_b) += phi / (N Q) * blk_b for every block of a chain
+__global__ void __launch_bounds__(NTA) k_block_rho(const double* __restrict__ blk, long m,
+                                                   SimpsonArgs a, const double* __restrict__ Qp,
+                                                   const double* __restrict__ phimu,
+                                                   double* __restrict__ rho_grid) {
+  const double Q = Qp[0];
+  const double phi = a.ensemble == 1 ? Q * exp(phimu[1]) : phimu[0];
+  for (long r = blockIdx.x * (long)NTA + threadIdx.x; r < m; r += (long)gridDim.x * NTA)
+    for (int b = 0; b < a.nblk; ++b) {
+      double* rg = rho_grid + (long)a.monomer[b] * m;
+      rg[r] = rg[r] + phi * (blk[(long)b * m + r] / a.chain_length / Q);
+    }
+}
+
 // workgroup partials of sum x*y
 __global__ void __launch_bounds__(NTA) k_partial_dot(const double* __restrict__ x,
                                                      const double* __restrict__ y, long n,

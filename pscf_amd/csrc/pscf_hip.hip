@@ -279,8 +279,32 @@ struct ChainDev {
   double ds[MAXBLK] = {0};
   int mon[MAXBLK] = {0};   // 0-based
   double length = 0;
-  DevBuf<double> qf, qr;   // (ns+1) * m
+  DevBuf<double> qf, qr;   // stored contour points: (ns+1) * m, or nslot * m when checkpointed
   DevBuf<double> fac;      // per block: e1, e2 (m), k1, k2 (mk)
+  // checkpointed storage (system stride K > 0): slot of every stored point
+  // (s % K == 0 or (ns - s) % K == 0), -1 for the others
+  std::vector<int> slot;
+  int nslot = 0;
+  DevBuf<int> dslot;
+  DevBuf<double> seg;      // recomputed segments: 4 x (K+1) points (f, r; two sides)
+  DevBuf<double> blk;      // per block: Simpson integral of the last density() (m each)
+  const double* point(int dir, int sp) const {    // stored contour point, or null
+    const int sl = slot.empty() ? sp : slot[sp];
+    return sl < 0 ? nullptr : (dir > 0 ? qf.p : qr.p) + (size_t)sl * m_;
+  }
+  long m_ = 0;
+};
+
+// Segment schedule of the checkpointed mode for stride K: round r recomputes,
+// per chain, the "forward-side" segment R_r = [rK, min((r+1)K, ns)] and the
+// "backward-side" segment F_r = [max(0, ns-(r+1)K), ns-rK] from checkpoints,
+// into the chain's segment buffers; maps[(r*nchain + c)*2 + side] maps a
+// contour point of the chain to its slot in that segment (-1 outside).
+struct CkPlan {
+  int nround = 0;
+  std::vector<int> len;          // per round: longest segment over the chains
+  DevBuf<int> maps;
+  std::vector<size_t> map_off;   // per (round, chain, side) offset into maps
 };
 
 // Adjoint (reverse-mode) sweep state of a system: per chain four sweep tasks
@@ -300,6 +324,11 @@ struct AdjState {
   DevBuf<StepTask> tasks;
   DevBuf<int2> pairs;
   DevBuf<double> part;     // reduction partials, and one result slot at the end
+  // checkpointed storage: per round the recomputation tasks (4 per chain) and
+  // the adjoint tasks re-pointed at that round's segments (4 per chain)
+  DevBuf<StepTask> ck_rec, ck_adj;
+  DevBuf<double2> rec_abc;
+  DevBuf<double> rec_q1;
   std::vector<StepTask> h_tasks;
   std::vector<size_t> blk_off;   // per chain: first block in gblk/kblk
   hipGraph_t graph = nullptr;
@@ -360,6 +389,19 @@ struct System {
     graph = nullptr;
   }
   std::unique_ptr<AdjState> adj;
+  // checkpointed contour storage
+  int ck_stride = 0;
+  double chain_step = 0.0;
+  CkPlan ck;
+  DevBuf<StepTask> ck_tasks;     // Simpson recomputation tasks: per round 2 per chain
+  hipGraph_t ck_graph = nullptr;
+  hipGraphExec_t ck_exec = nullptr;
+  void drop_ck_graph() {
+    if (ck_exec) (void)hipGraphExecDestroy(ck_exec);
+    if (ck_graph) (void)hipGraphDestroy(ck_graph);
+    ck_exec = nullptr;
+    ck_graph = nullptr;
+  }
   // stress work
   DevBuf<double> sbuf;     // real chunk
   DevBuf<double2> skbuf;   // spectra chunk
@@ -560,6 +602,98 @@ void grid_to_basis(System& S, const double* d_grid, int nf, double scale, double
                      scale, d_basis, 0);
 }
 
+// ---- checkpointed contour storage (stride K > 0)
+// The sweeps of density() keep q_f and q_r only at checkpoints; the Simpson
+// integrals and the adjoint recompute the points between them, one segment of
+// K steps per chain and round, into the chain's segment buffers.
+
+void build_ck_plan(System& S) {
+  const int K = S.ck_stride;
+  const int nc = S.nchain;
+  CkPlan& P = S.ck;
+  P.nround = (S.max_ns + K - 1) / K;
+  P.len.assign(P.nround, 0);
+  P.map_off.assign((size_t)P.nround * nc * 2, 0);
+  std::vector<int> h;
+  for (int r = 0; r < P.nround; ++r)
+    for (int c = 0; c < nc; ++c) {
+      const ChainDev& ch = *S.chains[c];
+      for (int side = 0; side < 2; ++side) {
+        int lo, hi;     // side 0: R_r, side 1: F_r
+        if (side == 0) {
+          lo = r * K;
+          hi = std::min((r + 1) * K, ch.ns);
+        } else {
+          hi = ch.ns - r * K;
+          lo = std::max(0, ch.ns - (r + 1) * K);
+        }
+        P.map_off[((size_t)r * nc + c) * 2 + side] = h.size();
+        for (int sp = 0; sp <= ch.ns; ++sp) h.push_back(lo <= hi && sp >= lo && sp <= hi ? sp - lo : -1);
+        if (lo < hi) P.len[r] = std::max(P.len[r], hi - lo);
+      }
+    }
+  P.maps.upload(h.data(), h.size(), S.st.s);
+  // forward Simpson tasks: per round, recompute R_r of every chain (q_f from
+  // its checkpoint at lo, q_r from its checkpoint at hi) into segment buffers 0, 1
+  std::vector<StepTask> ht((size_t)P.nround * 2 * nc);
+  for (int r = 0; r < P.nround; ++r)
+    for (int c = 0; c < nc; ++c) {
+      const ChainDev& ch = *S.chains[c];
+      const int lo = r * K, hi = std::min((r + 1) * K, ch.ns);
+      for (int d = 0; d < 2; ++d) {
+        StepTask t = S.h_tasks[2 * c + d];
+        StepTask& o = ht[((size_t)r * nc + c) * 2 + d];
+        t.init_one = 0;
+        t.slab = ch.seg.p + (size_t)d * (K + 1) * S.g.m;
+        t.pmap = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 0];
+        if (lo >= hi) {
+          t.skip = 1;
+        } else if (d == 0) {          // q_f forward over [lo, hi]
+          t.jbeg = t.jofs = lo;
+          t.jend = hi;
+          t.start = ch.point(+1, lo);
+        } else {                      // q_r backward over [lo, hi]: steps ns-hi .. ns-lo-1
+          t.jbeg = t.jofs = ch.ns - hi;
+          t.jend = ch.ns - lo;
+          t.start = ch.point(-1, hi);
+        }
+        o = t;
+      }
+    }
+  S.ck_tasks.upload(ht.data(), ht.size(), S.st.s);
+  S.st.sync();
+}
+
+// Simpson integrals of every block by segment recomputation (captured once)
+void ck_simpson_rounds(System& S, hipStream_t s) {
+  const Grid& g = S.g;
+  const CkPlan& P = S.ck;
+  const int nc = S.nchain, K = S.ck_stride;
+  for (int c = 0; c < nc; ++c) {
+    ChainDev& ch = *S.chains[c];
+    hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)ch.nblk * g.m)), dim3(NTA), 0, s, ch.blk.p,
+                       (long)ch.nblk * g.m, 0.0);
+  }
+  for (int r = 0; r < P.nround; ++r) {
+    const StepTask* tk = S.ck_tasks.p + (size_t)r * 2 * nc;
+    run_sweeps(g, tk, 2 * nc, P.len[r], s);
+    for (int c = 0; c < nc; ++c) {
+      ChainDev& ch = *S.chains[c];
+      const int lo = r * K, hi = std::min((r + 1) * K, ch.ns);
+      if (lo >= hi) continue;
+      SimpsonArgs a;
+      std::memset(&a, 0, sizeof a);
+      a.nblk = ch.nblk;
+      for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
+      for (int b = 0; b < ch.nblk; ++b) a.ds[b] = ch.ds[b];
+      hipLaunchKernelGGL(k_simpson_seg, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.seg.p,
+                         ch.seg.p + (size_t)(K + 1) * g.m, g.m, a, lo, hi, hi == ch.ns ? 1 : 0,
+                         ch.blk.p);
+    }
+  }
+  HIPCHK(hipGetLastError());
+}
+
 void startup(System& S, double chain_step) {
   REQUIRE(S.have_chem, "set_chemistry must precede startup");
   const Grid& g = S.g;
@@ -580,11 +714,25 @@ void startup(System& S, double chain_step) {
     ch->ns = ch->bnd[ch->nblk];
     ch->length = 0.0;
     for (int b = 0; b < ch->nblk; ++b) ch->length = ch->length + L[b];  // chemistry_mod.f90:560-566
-    ch->qf.alloc((size_t)(ch->ns + 1) * g.m);
-    ch->qr.alloc((size_t)(ch->ns + 1) * g.m);
+    ch->m_ = g.m;
+    if (S.ck_stride > 0) {
+      const int K = S.ck_stride;
+      ch->slot.assign(ch->ns + 1, -1);
+      ch->nslot = 0;
+      for (int sp = 0; sp <= ch->ns; ++sp)
+        if (sp % K == 0 || (ch->ns - sp) % K == 0) ch->slot[sp] = ch->nslot++;
+      ch->dslot.upload(ch->slot.data(), ch->slot.size(), S.st.s);
+      ch->seg.alloc((size_t)4 * (K + 1) * g.m);
+      ch->blk.alloc((size_t)ch->nblk * g.m);
+    } else {
+      ch->nslot = ch->ns + 1;
+    }
+    ch->qf.alloc((size_t)ch->nslot * g.m);
+    ch->qr.alloc((size_t)ch->nslot * g.m);
     ch->fac.alloc((size_t)ch->nblk * (2 * g.m + 2 * g.mk));
     S.chains.push_back(std::move(ch));
   }
+  S.chain_step = chain_step;
   const int ntask = 2 * S.nchain;
   S.abc.alloc((size_t)3 * g.mk * std::max(ntask, 1));
   S.q1.alloc((size_t)g.m * std::max(ntask, 1));
@@ -604,6 +752,7 @@ void startup(System& S, double chain_step) {
       StepTask& t = ht[2 * c + d];
       std::memset(&t, 0, sizeof t);
       t.slab = d == 0 ? ch.qf.p : ch.qr.p;
+      t.pmap = S.ck_stride > 0 ? ch.dslot.p : nullptr;
       t.ns = ch.ns;
       t.jend = ch.ns;
       t.dir = d == 0 ? 1 : -1;
@@ -632,9 +781,11 @@ void startup(System& S, double chain_step) {
   S.ntask = ntask;
   S.max_ns = max_ns;
   S.drop_graph();
+  S.drop_ck_graph();
   S.adj.reset();
   S.started = true;
   S.have_density = false;
+  if (S.ck_stride > 0) build_ck_plan(S);
 }
 
 // ---- density() in phases, each enqueued on stream s (src/scf_mod.fp.f90:144-229)
@@ -706,7 +857,7 @@ void density_post(System& S, double* rho, hipStream_t s) {
   const Grid& g = S.g;
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
-    dev_sum(ch.qf.p + (size_t)ch.ns * g.m, g.m, 1.0 / (double)g.m, S.part.p, S.qdev.p + c, s);
+    dev_sum(ch.point(+1, ch.ns), g.m, 1.0 / (double)g.m, S.part.p, S.qdev.p + c, s);
   }
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)S.nmon * g.m)), dim3(NTA), 0, s, S.rho_grid.p,
                      (long)S.nmon * g.m, 0.0);
@@ -720,6 +871,19 @@ void density_post(System& S, double* rho, hipStream_t s) {
                        S.qdev.p + S.nchain + i, S.ensemble, S.chem_dev.p + 2 * (S.nchain + i), g.m,
                        S.rho_grid.p + (size_t)S.solv_mon[i] * g.m);
   }
+  if (S.ck_stride > 0) {
+    if (S.use_graph) {
+      if (!S.ck_exec) {
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        ck_simpson_rounds(S, s);
+        HIPCHK(hipStreamEndCapture(s, &S.ck_graph));
+        HIPCHK(hipGraphInstantiate(&S.ck_exec, S.ck_graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(S.ck_exec, s));
+    } else {
+      ck_simpson_rounds(S, s);
+    }
+  }
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
     SimpsonArgs a;
@@ -732,8 +896,12 @@ void density_post(System& S, double* rho, hipStream_t s) {
     }
     a.chain_length = ch.length;
     a.ensemble = S.ensemble;
-    hipLaunchKernelGGL(k_simpson, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a,
-                       S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
+    if (S.ck_stride > 0)
+      hipLaunchKernelGGL(k_block_rho, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.blk.p, g.m, a,
+                         S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
+    else
+      hipLaunchKernelGGL(k_simpson, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m,
+                         a, S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
   }
   const size_t nst = (size_t)S.nmon * S.nstar;
   if (rho) {
@@ -890,32 +1058,100 @@ void adjoint_setup(System& S, bool with_k) {
                         S.st.s));
   HIPCHK(hipMemcpyAsync(A->pairs.p, hp.data(), sizeof(int2) * A->npair, hipMemcpyHostToDevice,
                         S.st.s));
+  if (S.ck_stride > 0) {
+    // segment buffers of a chain: 0 q_f(R_r), 1 q_r(R_r), 2 q_f(F_r), 3 q_r(F_r)
+    const int K = S.ck_stride, nc = S.nchain;
+    const CkPlan& P = S.ck;
+    A->rec_abc.alloc((size_t)3 * g.mk * 4 * nc);
+    A->rec_q1.alloc((size_t)g.m * 4 * nc);
+    std::vector<StepTask> rec((size_t)P.nround * 4 * nc), adj((size_t)P.nround * 4 * nc);
+    for (int r = 0; r < P.nround; ++r)
+      for (int c = 0; c < nc; ++c) {
+        ChainDev& ch = *S.chains[c];
+        auto seg = [&](int i) { return ch.seg.p + (size_t)i * (K + 1) * g.m; };
+        const int* mapR = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 0];
+        const int* mapF = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 1];
+        // adjoint tasks of this round; a chain whose steps ended in an earlier
+        // round (its last finishing step may fall here) keeps that round's segments
+        const int rl = std::min(r, (ch.ns + K - 1) / K - 1);
+        const int* amapR = P.maps.p + P.map_off[((size_t)rl * nc + c) * 2 + 0];
+        const int* amapF = P.maps.p + P.map_off[((size_t)rl * nc + c) * 2 + 1];
+        for (int k = 0; k < 4; ++k) {
+          StepTask t = ht[4 * c + k];
+          if (k == 0) { t.oth = seg(3); t.omap = amapF; }         // a_f: sources from q_r on F_r
+          if (k == 1) { t.slab = seg(2); t.pmap = amapF; }        // q_f recomputed on F_r
+          if (k == 2) { t.oth = seg(0); t.omap = amapR; }         // a_r: sources from q_f on R_r
+          if (k == 3) { t.slab = seg(1); t.pmap = amapR; }        // q_r recomputed on R_r
+          adj[((size_t)r * nc + c) * 4 + k] = t;
+        }
+        // recomputation tasks: q_f and q_r on R_r (buffers 0, 1) and on F_r (2, 3)
+        for (int k = 0; k < 4; ++k) {
+          const int side = k / 2, d = k % 2;         // d 0: q_f forward, 1: q_r backward
+          int lo, hi;
+          if (side == 0) { lo = r * K; hi = std::min((r + 1) * K, ch.ns); }
+          else { hi = ch.ns - r * K; lo = std::max(0, ch.ns - (r + 1) * K); }
+          StepTask t = S.h_tasks[2 * c + d];
+          t.init_one = 0;
+          t.slab = seg(2 * side + d);
+          t.pmap = side == 0 ? mapR : mapF;
+          const int ti = 4 * c + k;
+          t.A = A->rec_abc.p + (size_t)(3 * ti) * g.mk;
+          t.B = t.A + g.mk;
+          t.C = t.B + g.mk;
+          t.q1 = A->rec_q1.p + (size_t)ti * g.m;
+          if (lo >= hi) {
+            t.skip = 1;
+          } else if (d == 0) {
+            t.jbeg = t.jofs = lo;
+            t.jend = hi;
+            t.start = ch.point(+1, lo);
+          } else {
+            t.jbeg = t.jofs = ch.ns - hi;
+            t.jend = ch.ns - lo;
+            t.start = ch.point(-1, hi);
+          }
+          rec[((size_t)r * nc + c) * 4 + k] = t;
+        }
+      }
+    A->ck_rec.upload(rec.data(), rec.size(), S.st.s);
+    A->ck_adj.upload(adj.data(), adj.size(), S.st.s);
+  }
   S.st.sync();
   A->h_tasks = ht;
   S.adj = std::move(A);
 }
 
-// the adjoint sweeps with their gradient kernels (launch order as run_sweeps)
+// the adjoint sweeps with their gradient kernels (launch order as run_sweeps).
+// Checkpointed storage: step j runs with the tasks of round j / K, whose
+// segments the recomputation sweeps fill just before the round's first step.
 void run_adj_sweeps(const System& S, hipStream_t s) {
   const Grid& g = S.g;
   const AdjState& A = *S.adj;
   const int nt = A.ntask, max_ns = S.max_ns;
   const int gb = grid_blocks(g.m), gk = grid_blocks(g.mk);
+  const int K = S.ck_stride;
+  const StepTask* tasks = A.tasks.p;
   for (int j = 0; j <= max_ns; ++j) {
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
+    if (K > 0) {
+      const int r = std::min(j / K, S.ck.nround - 1);
+      if (j % K == 0 && j / K < S.ck.nround)
+        run_sweeps(g, A.ck_rec.p + (size_t)r * nt, nt, S.ck.len[r], s);
+      tasks = A.ck_adj.p + (size_t)r * nt;
+    }
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, tasks, j, g.n3, g.tw1,
                        g.tw2);
     if (j >= 1)
-      hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, A.tasks.p, A.pairs.p, A.npair,
+      hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, tasks, A.pairs.p, A.npair,
                          j - 1, g.m);
     if (j == max_ns) break;
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 2), dim3(g.zs->nt_step), 0, s, A.tasks.p, j, g.n2, 0,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 2), dim3(g.zs->nt_step), 0, s, tasks, j, g.n2, 0,
                        g.tw3);
-    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, A.tasks.p, j, g.n3, g.tw1,
+    hipLaunchKernelGGL(g.pl->c, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, tasks, j, g.n3, g.tw1,
                        g.tw2);
-    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 1), dim3(g.zs->nt_step), 0, s, A.tasks.p, j, g.n2, 1,
+    hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 1), dim3(g.zs->nt_step), 0, s, tasks, j, g.n2, 1,
                        g.tw3);
     if (A.with_k)
-      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk), dim3(256), 0, s, A.tasks.p, A.pairs.p, A.npair, j,
+      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk), dim3(256), 0, s, tasks, A.pairs.p, A.npair, j,
                          g.n1, g.mk, (double)g.m);
   }
   HIPCHK(hipGetLastError());
@@ -964,10 +1200,18 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
     }
     a.chain_length = ch.length;
     a.ensemble = S.ensemble;
-    hipLaunchKernelGGL(k_simpson_lam, dim3(nb, ch.nblk), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a, lam,
-                       part);
     T[c].resize(ch.nblk);
-    for (int b = 0; b < ch.nblk; ++b) T[c][b] = host_sum(S, part + (size_t)b * nb, nb);
+    if (S.ck_stride > 0) {      // the block integrals of the last density() are kept
+      for (int b = 0; b < ch.nblk; ++b) {
+        hipLaunchKernelGGL(k_partial_dot, dim3(nb), dim3(NTA), 0, s, ch.blk.p + (size_t)b * g.m,
+                           lam + (size_t)ch.mon[b] * g.m, g.m, part);
+        T[c][b] = host_sum(S, part, nb);
+      }
+    } else {
+      hipLaunchKernelGGL(k_simpson_lam, dim3(nb, ch.nblk), dim3(NTA), 0, s, ch.qf.p, ch.qr.p, g.m, a,
+                         lam, part);
+      for (int b = 0; b < ch.nblk; ++b) T[c][b] = host_sum(S, part + (size_t)b * nb, nb);
+    }
     const double Q = S.h_Q[c];
     coef[c] = S.ensemble == 1 ? std::exp(S.h_chem[2 * c + 1]) / ch.length
                               : S.h_chem[2 * c] / (ch.length * Q);
@@ -1064,6 +1308,7 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
 // (:505-528) on the device, copied to the host (S.h_dots)
 void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
   REQUIRE(S.have_density, "scf_stress requires a previous density()");
+  REQUIRE(S.ck_stride == 0, "scf_stress needs every contour point (checkpointed storage is on)");
   const Grid& g = S.g;
   S.dgsq.upload(dgsq, (size_t)S.nstar * ncp, s);
   const int chunk = 32;
@@ -1580,6 +1825,7 @@ int pscf_system_destroy(int64_t sys) {
     System* S = from_handle<System>(sys, "system");
     for (auto e : S->ev) (void)hipEventDestroy(e);
     S->drop_graph();
+    S->drop_ck_graph();
     delete S;
   });
 }
@@ -1598,11 +1844,23 @@ int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count) {
   });
 }
 
+int pscf_system_set_checkpoint(int64_t sys, int stride) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(stride >= 0, "checkpoint stride must be >= 0");
+    S.ck_stride = stride;
+    if (S.started) startup(S, S.chain_step);
+  });
+}
+
 int pscf_system_use_graph(int64_t sys, int enable) {
   return guarded([&] {
     System& S = *from_handle<System>(sys, "system");
     S.use_graph = enable != 0;
-    if (!S.use_graph) S.drop_graph();
+    if (!S.use_graph) {
+      S.drop_graph();
+      S.drop_ck_graph();
+    }
   });
 }
 
@@ -1870,6 +2128,7 @@ int pscf_divide_energy_terms(int64_t sys, double* out) {
   return guarded([&] {
     System& S = *from_handle<System>(sys, "system");
     REQUIRE(S.have_density, "divide_energy requires a previous density()");
+    REQUIRE(S.ck_stride == 0, "divide_energy needs every contour point (checkpointed storage is on)");
     const Grid& g = S.g;
     for (int c = 0; c < S.nchain; ++c) {
       ChainDev& ch = *S.chains[c];
@@ -1906,9 +2165,11 @@ int pscf_system_get_q(int64_t sys, int chain, int dir, int s, double* q) {
     System& S = *from_handle<System>(sys, "system");
     REQUIRE(S.have_density, "no density computed yet");
     REQUIRE(chain >= 0 && chain < S.nchain, "chain index out of range");
+    REQUIRE(S.ck_stride == 0 || S.chains[chain]->slot[s] >= 0,
+            "contour point not stored (checkpointed storage)");
     ChainDev& ch = *S.chains[chain];
     REQUIRE(s >= 0 && s <= ch.ns, "contour index out of range");
-    const double* src = (dir > 0 ? ch.qf.p : ch.qr.p) + (size_t)s * S.g.m;
+    const double* src = ch.point(dir > 0 ? 1 : -1, s);
     HIPCHK(hipMemcpy(q, src, sizeof(double) * S.g.m, hipMemcpyDeviceToHost));
   });
 }

@@ -258,6 +258,11 @@ class SCFSystem:
                       n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
         return ms, n
 
+    def set_checkpoint(self, stride):
+        """Keep q and q-dagger only every `stride` contour points (0: all points);
+        the Simpson integrals and the adjoint recompute the rest."""
+        self.lib.call("pscf_system_set_checkpoint", self.h, int(stride))
+
     def use_graph(self, enable):
         self.lib.call("pscf_system_use_graph", self.h, int(bool(enable)))
 

@@ -198,6 +198,56 @@ def test_backward_after_new_forward_raises(backend):
         rho.sum().backward()
 
 
+def _loss_and_grads(S, s, omg, dev, seed=11):
+    rng = np.random.default_rng(seed)
+    tgt = torch.tensor(rng.normal(0.3, 0.05, omg.shape[1:]), device=dev)
+    om = omg.clone().requires_grad_(True)
+    cp = torch.tensor(np.array(s.cell_param, dtype=np.float64), requires_grad=True)
+    Lb = adjoint.block_lengths(s).requires_grad_(True)
+    rho, Q = adjoint.density(S, om, cp, Lb)
+    loss = ((rho[0] - tgt) ** 2).sum()
+    for c in range(s.N_chain):
+        loss = loss + 0.3 * (c + 1) * torch.log(Q[c])
+    loss.backward()
+    return [x.detach().cpu().numpy() for x in (rho, Q, om.grad, cp.grad, Lb.grad)]
+
+
+@pytest.mark.parametrize("name,step,stride", [("bcc8", None, 3), ("blend_nosolv", 0.09, 3)])
+def test_checkpointed_storage_matches_full(backend, name, step, stride):
+    """q and q-dagger kept only every `stride` contour points (and at the
+    points counted from Ns), the rest recomputed per segment: density and every
+    gradient equal those of full storage.  The blend's chains have different
+    Ns, and stride 3 ends the homopolymer exactly on a segment boundary."""
+    S, s, g, omg, dev = _make(name, backend, step)
+    full = _loss_and_grads(S, s, omg, dev)
+    S2, s2, g2, omg2, dev2 = _make(name, backend, step)
+    S2.set_checkpoint(stride)
+    ck = _loss_and_grads(S2, s2, omg2, dev2)
+    for nm, a, b, tol in zip(["rho", "Q", "grad_omega", "grad_cell", "grad_length"], ck, full,
+                             [1e-14, 1e-14, 1e-11, 1e-11, 1e-10]):
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), (nm, np.abs(a - b).max(), np.abs(b).max())
+    with pytest.raises(Exception, match="every contour point"):
+        S2.scf_stress(np.zeros((S2.N_star, 1)))
+
+
+@pytest.mark.gpu
+def test_gyroid64_checkpoint_stride14_gpu():
+    """The gyroid 64^3 gradient with q checkpointed every 14 of the 200 contour
+    steps equals the full-storage gradient."""
+    lib = get_lib("gpu")
+    s, g = _system("gyr64")
+    out = []
+    for stride in (0, 14):
+        S = batch.make_system(s, g, lib=lib)
+        if stride:
+            S.set_checkpoint(stride)
+        omg = adjoint.stars_to_grid(S, torch.tensor(g["omega"], device="cuda")).detach()
+        out.append(_loss_and_grads(S, s, omg, "cuda"))
+    for nm, a, b, tol in zip(["rho", "Q", "grad_omega", "grad_cell", "grad_length"], out[1], out[0],
+                             [1e-14, 1e-14, 1e-11, 1e-11, 1e-10]):
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), (nm, np.abs(a - b).max())
+
+
 @pytest.mark.gpu
 def test_gyroid64_loss_gradients_gpu():
     """The gyroid 64^3 system (N=100, f=0.36, chi N=20, a=16, ds=0.5): loss
