@@ -29,10 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by the same counters on a plain copy of known size); the summary and
-# its commands are in profiles/.  None until measured.
-TRAFFIC = {}
+# HBM bytes per launch of each step kernel at the gyroid 64^3 workload (both
+# sweeps), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# `bench.py --traffic-probe`, each scaled by the same counter on a 256 MiB
+# device copy (reads 2.0 x FETCH_SIZE, writes 1.0 x WRITE_SIZE):
+# profiles/k_step_gyroid64_hbm_traffic.md
+TRAFFIC = {"step_a": 28.80e6, "step_z0": 24.91e6, "step_c": 20.97e6, "step_z1": 12.51e6}
 WORKLOAD = "gyroid_Ia-3d_64^3_N100_f0.36_chiN20_ds0.5_a16_AM_domain"
 
 
@@ -103,12 +105,15 @@ def cpu_baseline(g, n_gpu_iter, workdir):
 # 64 AB diblock melts (N=100, ds=1) with chi N ~ U[12, 30], f_A ~ U[0.2, 0.5]
 # and a phase drawn per instance: lamellae in a tetragonal cell on 32^3,
 # hexagonal cylinders (3D cell) on 48^3, BCC or gyroid on 64^3.  Each runs
-# its own Anderson history (fixed cell) to residual 1e-8.
-BATCH_PHASES = {
-    "lam": ("lam_tet32", "tetragonal", [16.0, 16.0], [32, 32, 32], "P 4/m m m"),
-    "hex": ("hex48", "hexagonal", [19.0, 19.0], [48, 48, 48], "P 6/m m m"),
-    "bcc": ("bcc64", "cubic", [18.0], [64, 64, 64], "I m -3 m"),
-    "gyr": ("gyr64", "cubic", [36.0], [64, 64, 64], "I a -3 d"),
+# its own Anderson history (fixed cell) to residual 1e-8.  Starting field:
+# the phase's principal star (lamellae: (001), cylinders: (100), BCC: (110),
+# gyroid: (211)) at amplitude +-chi/4 in omega_A/omega_B, and the other
+# stars 2..10 seeded N(0, chi/20).
+BATCH_PHASES = {   # tables, crystal system, cell, grid, group, principal star
+    "lam": ("lam_tet32", "tetragonal", [16.0, 16.0], [32, 32, 32], "P 4/m m m", 2),
+    "hex": ("hex48", "hexagonal", [19.0, 19.0], [48, 48, 48], "P 6/m m m", 2),
+    "bcc": ("bcc64", "cubic", [18.0], [64, 64, 64], "I m -3 m", 1),
+    "gyr": ("gyr64", "cubic", [36.0], [64, 64, 64], "I a -3 d", 1),
 }
 BATCH_SEED = 20261203
 
@@ -122,7 +127,7 @@ def batch_instances(n=64, seed=BATCH_SEED):
         ph = ["lam", "hex", "bcc", "gyr"][int(rng.integers(4))]
         chiN = float(rng.uniform(12.0, 30.0))
         fA = float(rng.uniform(0.2, 0.5))
-        tname, cs, cell, grid, group = BATCH_PHASES[ph]
+        tname, cs, cell, grid, group, principal = BATCH_PHASES[ph]
         if tname not in tables:
             d = np.load(os.path.join(ROOT, "tests", "golden", tname + ".npz"))
             tables[tname] = {k: d[k] for k in d.files}
@@ -130,13 +135,16 @@ def batch_instances(n=64, seed=BATCH_SEED):
         sysm = diblock(fA, 100.0, chiN, dim=3, crystal_system=cs, cell_param=cell, ngrid=grid,
                        chain_step=1.0, group_name=group)
         nstar = len(g["star_count"])
+        chi = chiN / 100.0
         om = np.zeros((2, nstar))
-        om[:, 1:10] = rng.normal(0.0, 0.5 * chiN / 100.0, size=(2, 9))
+        om[:, 1:10] = rng.normal(0.0, 0.05 * chi, size=(2, 9))
+        om[0, principal] = 0.25 * chi
+        om[1, principal] = -0.25 * chi
         out.append((ph, sysm, g, om))
     return out
 
 
-def batch_leg(L, n_inst=64, n_loop=16, error_max=1e-8, max_itr=500):
+def batch_leg(L, n_inst=64, n_loop=16, error_max=1e-8, max_itr=300):
     """Converged solves/s of the batched solver over all instances, and of a
     per-instance loop (the plain single-system path) over every (n_inst /
     n_loop)-th instance."""
@@ -183,8 +191,8 @@ def batch_leg(L, n_inst=64, n_loop=16, error_max=1e-8, max_itr=500):
             "loop_seconds": t_loop, "speedup_vs_loop": (v_b / v_l) if v_l else None,
             "loop_matches_batch": same,
             "workload": "64 AB diblocks N=100 ds=1, chiN~U[12,30], fA~U[0.2,0.5], "
-                        "phases lam(32^3)/hex(48^3)/bcc,gyr(64^3), AM n_hist=20, fixed cell, "
-                        "residual 1e-8, seed %d" % BATCH_SEED}
+                        "phases lam(32^3)/hex(48^3)/bcc,gyr(64^3) from their principal star, "
+                        "AM n_hist=20, fixed cell, residual 1e-8, max_itr 300, seed %d" % BATCH_SEED}
 
 
 # --------------------------------------------------------------- adjoint leg
@@ -304,18 +312,18 @@ def main():
     # whole captured sweeps (slot 4): milliseconds and contour steps advanced
     steps_timed = int(n[4])
     t_step_ms = float(ms[4]) / max(steps_timed, 1)
-    # per-kernel durations: HIP events around every step kernel on the system
-    # stream, over two density() calls at the converged field
-    S.profile_sweeps(2)
-    for _ in range(2):
+    # per-kernel durations: each step kernel launched 100 times back to back
+    # at contour step 100 on the stream the sweeps use, between two HIP events,
+    # at the converged field
+    names = ["step_a", "step_z0", "step_c", "step_z1"]
+    S.density(r.omega)
+    kavg_us = {}
+    for i, nm in enumerate(names):
+        kavg_us[nm] = S.profile_step_kernel(i, 100, reps=100)
         S.density(r.omega)
     torch.cuda.synchronize()
-    kms, kn = S.profile_sweeps(0)
-    names = ["step_a", "step_z0", "step_c", "step_z1"]
-    kavg_us = {nm: float(kms[i]) / max(int(kn[i]), 1) * 1e3 for i, nm in enumerate(names)}
-    ktot = {nm: float(kms[i]) for i, nm in enumerate(names)}
     kb = {nm: 2 * v for nm, v in kernel_bytes(M, Mk).items()}   # qf and qr sweeps per launch
-    dom = max(names, key=lambda nm: ktot[nm])
+    dom = max(names, key=kavg_us.get)           # each kernel runs once per contour step
     achieved = kb[dom] / (kavg_us[dom] * 1e-6) / 1e9
     bytes_step = sum(kb.values())
 

@@ -167,6 +167,11 @@ int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count);
  * startup when the system is started.  scf_stress and divide_energy need full
  * storage and fail while it is on. */
 int pscf_system_set_checkpoint(int64_t sys, int stride);
+/* Average duration (us) of one step kernel (kind 0 k_step_a, 1 k_step_z on
+ * A and B, 2 k_step_c, 3 k_step_z on C) launched `reps` times back to back at
+ * contour step j (1 <= j < Ns) on the tasks of the last density().  The z
+ * kernels transform in place, so the system needs a new density() after. */
+int pscf_profile_step_kernel(int64_t sys, int kind, int j, int reps, double* us);
 /* Capture the sweeps of density() in a HIP graph (default on). */
 int pscf_system_use_graph(int64_t sys, int enable);
 

@@ -64,6 +64,8 @@ SIGNATURES = {
     "pscf_profile_sweeps": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_i64_p],
     "pscf_system_use_graph": [ctypes.c_int64, ctypes.c_int],
     "pscf_system_set_checkpoint": [ctypes.c_int64, ctypes.c_int],
+    "pscf_profile_step_kernel": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _c_dbl_p],
     "pscf_basis_to_kgrid": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_kgrid_to_basis": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_batch_create": [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_i64_p],

@@ -1830,6 +1830,39 @@ int pscf_system_destroy(int64_t sys) {
   });
 }
 
+int pscf_profile_step_kernel(int64_t sys, int kind, int j, int reps, double* us) {
+  return guarded([&] {
+    System& S = *from_handle<System>(sys, "system");
+    REQUIRE(S.have_density, "profile_step_kernel requires a previous density()");
+    REQUIRE(kind >= 0 && kind <= 3 && reps >= 1, "kind must be 0..3, reps >= 1");
+    REQUIRE(j >= 1 && j < S.max_ns, "step index out of range");
+    const Grid& g = S.g;
+    hipStream_t s = S.st.s;
+    ensure_events(S, 2);
+    auto launch = [&] {
+      if (kind == 0)
+        hipLaunchKernelGGL(g.pl->a, dim3(g.n3, S.ntask, 2), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
+                           g.tw1, g.tw2);
+      else if (kind == 2)
+        hipLaunchKernelGGL(g.pl->c, dim3(g.n3, S.ntask, 2), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
+                           g.tw1, g.tw2);
+      else
+        hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, S.ntask, kind == 1 ? 2 : 1),
+                           dim3(g.zs->nt_step), 0, s, S.tasks.p, j, g.n2, kind == 1 ? 0 : 1, g.tw3);
+    };
+    launch();
+    HIPCHK(hipEventRecord(S.ev[0], s));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHK(hipEventRecord(S.ev[1], s));
+    HIPCHK(hipGetLastError());
+    S.st.sync();
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+    *us = (double)ms * 1e3 / reps;
+    S.have_density = false;      // the z kernels transform their spectra in place
+  });
+}
+
 int pscf_profile_sweeps(int64_t sys, int mode, double* ms_sum, int64_t* count) {
   return guarded([&] {
     System& S = *from_handle<System>(sys, "system");

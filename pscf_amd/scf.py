@@ -258,6 +258,15 @@ class SCFSystem:
                       n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
         return ms, n
 
+    def profile_step_kernel(self, kind, j, reps=50):
+        """Average microseconds of step kernel `kind` (0 k_step_a, 1 k_step_z on A/B,
+        2 k_step_c, 3 k_step_z on C) launched back to back at step j; call
+        density() again afterwards."""
+        us = ctypes.c_double(0.0)
+        self.lib.call("pscf_profile_step_kernel", self.h, int(kind), int(j), int(reps),
+                      ctypes.byref(us))
+        return us.value
+
     def set_checkpoint(self, stride):
         """Keep q and q-dagger only every `stride` contour points (0: all points);
         the Simpson integrals and the adjoint recompute the rest."""
