@@ -1,8 +1,9 @@
 """Autotuner of the contour-step kernel plans (pscf_amd/csrc/tuned.hpp).
 
-The plan of a grid length n is (max_radix, xpad, zsplit): the largest Stockham
-radix of its transforms, the LDS padding of the XY-plane rows and the number
-of workgroups per ky row of the z-column step kernel.  They are compile-time
+The plan of a grid length n is (max_radix, xpad, zsplit, alias): the largest
+Stockham radix of its transforms, the LDS padding of the XY-plane rows, the
+number of workgroups per ky row of the z-column step kernel, and whether the
+packed-row and half-spectrum buffers of an XY plane share LDS.  They are compile-time
 parameters, so the tuner builds one library per candidate plan (overriding
 the table for every length), times the captured contour sweeps of a
 synthetic system on each candidate and grid length on the device, and writes
@@ -35,14 +36,16 @@ TUNED_HPP = os.path.join(HERE, "csrc", "tuned.hpp")
 RADIX = (8, 4)
 XPAD = (0, 1)
 ZSPLIT = (1, 2, 3)
+ALIAS = (0, 1)
 
 
 def candidates():
-    return [dict(max_radix=r, xpad=p, zsplit=z) for r, p, z in itertools.product(RADIX, XPAD, ZSPLIT)]
+    return [dict(max_radix=r, xpad=p, zsplit=z, alias=a)
+            for r, p, z, a in itertools.product(RADIX, XPAD, ZSPLIT, ALIAS)]
 
 
 def variant_name(c):
-    return "r%d_p%d_z%d" % (c["max_radix"], c["xpad"], c["zsplit"])
+    return "r%d_p%d_z%d_a%d" % (c["max_radix"], c["xpad"], c["zsplit"], c["alias"])
 
 
 def build(jobs=4):
@@ -52,7 +55,7 @@ def build(jobs=4):
     def one(c):
         out = os.path.join(TUNE_DIR, "lib_%s.so" % variant_name(c))
         build_lib(defines={"PSCF_MAX_RADIX": c["max_radix"], "PSCF_XPAD": c["xpad"],
-                           "PSCF_ZSPLIT": c["zsplit"]}, out=out)
+                           "PSCF_ZSPLIT": c["zsplit"], "PSCF_ALIAS": c["alias"]}, out=out)
         return out
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -123,21 +126,23 @@ def write(results=RESULTS):
     rows = []
     for n in sorted(res, key=int):
         best = min(res[n], key=res[n].get)
-        r, p, z = [int(t[1:]) for t in best.split("_")]
-        rows.append((int(n), r, p, z, res[n][best], res[n].get("r8_p0_z1")))
+        r, p, z, a = [int(t[1:]) for t in best.split("_")]
+        rows.append((int(n), r, p, z, a, res[n][best], res[n].get("r8_p0_z1_a0")))
     lines = [
         "// Kernel plans per FFT/grid length for gfx950, written by pscf_amd/autotune.py",
         "// from timings of the contour sweeps on an MI355X (rerun it to regenerate).",
         "//   max_radix  largest Stockham radix of a transform of this length (8 or 4)",
         "//   xpad       LDS padding (complex words) per packed row of an XY plane",
         "//   zsplit     workgroups sharing one ky row of the z-column step kernel",
+        "//   alias      1: the packed-row and half-spectrum LDS buffers of an XY plane",
+        "//              share memory (half the LDS per workgroup, more workgroups per CU)",
         "#pragma once", "", "namespace pscf {", "",
-        "struct TunedPlan {", "  int n, max_radix, xpad, zsplit;", "};", "",
+        "struct TunedPlan {", "  int n, max_radix, xpad, zsplit, alias;", "};", "",
         "constexpr TunedPlan kTunedPlans[] = {"]
-    for n, r, p, z, us, base in rows:
-        note = "  // %.2f us/step (radix 8, no pad, 1 row: %s)" % (
+    for n, r, p, z, a, us, base in rows:
+        note = "  // %.2f us/step (radix 8, no pad, 1 row, separate: %s)" % (
             us, "%.2f" % base if base is not None else "n/a")
-        lines.append("    {%d, %d, %d, %d},%s" % (n, r, p, z, note))
+        lines.append("    {%d, %d, %d, %d, %d},%s" % (n, r, p, z, a, note))
     lines += ["};", "", "}  // namespace pscf", ""]
     open(TUNED_HPP, "w").write("\n".join(lines))
     print("wrote", TUNED_HPP)
@@ -148,7 +153,7 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["build", "measure", "write"])
     ap.add_argument("--sizes", default="32,48,64,96")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=6)
     ap.add_argument("--results", default=RESULTS, help="timing table (json)")
     a = ap.parse_args(argv)
     if a.cmd == "build":

@@ -32,13 +32,21 @@ template <int S> __device__ __forceinline__ double2 cmul_si(double2 a) {
 
 // ------------------------------------------------------ compile-time factoring
 // Per-length kernel plan: the tuned table (tuned.hpp) or, for lengths it does
-// not list, radix 8, no padding, one workgroup per z row.  PSCF_MAX_RADIX,
-// PSCF_XPAD and PSCF_ZSPLIT, when defined, override the plan for every length
-// (the variants the autotuner builds and times).
+// not list, radix 8, no padding, one workgroup per z row, separate plane
+// buffers.  PSCF_MAX_RADIX, PSCF_XPAD, PSCF_ZSPLIT and PSCF_ALIAS, when
+// defined, override the plan for every length (the variants the autotuner
+// builds and times).
 constexpr TunedPlan plan_for(int n) {
   for (const TunedPlan& p : kTunedPlans)
     if (p.n == n) return p;
-  return TunedPlan{n, 8, 0, 1};
+  return TunedPlan{n, 8, 0, 1, 0};
+}
+constexpr int plan_alias(int n) {
+#ifdef PSCF_ALIAS
+  return PSCF_ALIAS;
+#else
+  return plan_for(n).alias;
+#endif
 }
 constexpr int plan_max_radix(int n) {
 #ifdef PSCF_MAX_RADIX
@@ -239,9 +247,13 @@ template <int N1, int N2> struct Plane {
   static constexpr int PSIZE = PLS * NP;         // complex words in P
   static constexpr int HSIZE = N1H * N2;         // complex words in H
   static constexpr bool XJFAST = plan_xpad(N1) == 0;  // thread mapping of the x passes
+  static constexpr bool ALIAS = plan_alias(N1) != 0;   // P and H share one LDS buffer
+  static constexpr int USIZE = ALIAS ? (PSIZE > HSIZE ? PSIZE : HSIZE) : PSIZE + HSIZE;
   static_assert(N1 % 2 == 0, "first grid dimension must be even");
   // LDS index of element x of packed row pl, given i = pl*N1 + x
   static __device__ __forceinline__ int pidx(int i) { return (i / N1) * PLS + i % N1; }
+  static __device__ __forceinline__ double2* Pbuf(double2* U) { return U; }
+  static __device__ __forceinline__ double2* Hbuf(double2* U) { return ALIAS ? U : U + PSIZE; }
 };
 
 // P (x-transformed pairs) -> H (half rows)
@@ -249,14 +261,28 @@ template <int N1, int N2, int NT>
 __device__ __forceinline__ void split_pairs(const double2* P, double2* H, int tid) {
   using PL = Plane<N1, N2>;
   constexpr int TOT = PL::NP * PL::N1H;
-  for (int i = tid; i < TOT; i += NT) {
-    const int pl = i / PL::N1H, k = i % PL::N1H;
-    const double2 z = P[pl * PL::PLS + k];
-    const double2 zc = cconj(P[pl * PL::PLS + (N1 - k) % N1]);
-    H[(2 * pl) * PL::N1H + k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
-    if (2 * pl + 1 < N2) {
-      const double2 d = csub(z, zc);                       // (z - zc) / (2i)
-      H[(2 * pl + 1) * PL::N1H + k] = make_double2(0.5 * d.y, -0.5 * d.x);
+  constexpr int PER = (TOT + NT - 1) / NT;
+  double2 z[PER], zc[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) {
+      const int pl = i / PL::N1H, k = i % PL::N1H;
+      z[p] = P[pl * PL::PLS + k];
+      zc[p] = cconj(P[pl * PL::PLS + (N1 - k) % N1]);
+    }
+  }
+  if (PL::ALIAS) __syncthreads();                           // H overwrites P
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) {
+      const int pl = i / PL::N1H, k = i % PL::N1H;
+      H[(2 * pl) * PL::N1H + k] = make_double2(0.5 * (z[p].x + zc[p].x), 0.5 * (z[p].y + zc[p].y));
+      if (2 * pl + 1 < N2) {
+        const double2 d = csub(z[p], zc[p]);               // (z - zc) / (2i)
+        H[(2 * pl + 1) * PL::N1H + k] = make_double2(0.5 * d.y, -0.5 * d.x);
+      }
     }
   }
   __syncthreads();
@@ -267,15 +293,27 @@ template <int N1, int N2, int NT>
 __device__ __forceinline__ void join_pairs(const double2* H, double2* P, int tid) {
   using PL = Plane<N1, N2>;
   constexpr int TOT = PL::NP * N1;
-  for (int i = tid; i < TOT; i += NT) {
-    const int pl = i / N1, x = i % N1;
-    const bool hi = x > N1 / 2;
-    const int k = hi ? N1 - x : x;
-    double2 a = H[(2 * pl) * PL::N1H + k];
-    double2 b = (2 * pl + 1 < N2) ? H[(2 * pl + 1) * PL::N1H + k] : make_double2(0.0, 0.0);
-    if (k == 0 || k == N1 / 2) { a.y = 0.0; b.y = 0.0; }
-    if (hi) { a.y = -a.y; b.y = -b.y; }
-    P[pl * PL::PLS + x] = make_double2(a.x - b.y, a.y + b.x);     // a + i b
+  constexpr int PER = (TOT + NT - 1) / NT;
+  double2 v[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) {
+      const int pl = i / N1, x = i % N1;
+      const bool hi = x > N1 / 2;
+      const int k = hi ? N1 - x : x;
+      double2 a = H[(2 * pl) * PL::N1H + k];
+      double2 b = (2 * pl + 1 < N2) ? H[(2 * pl + 1) * PL::N1H + k] : make_double2(0.0, 0.0);
+      if (k == 0 || k == N1 / 2) { a.y = 0.0; b.y = 0.0; }
+      if (hi) { a.y = -a.y; b.y = -b.y; }
+      v[p] = make_double2(a.x - b.y, a.y + b.x);           // a + i b
+    }
+  }
+  if (PL::ALIAS) __syncthreads();                           // P overwrites H
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int i = tid + p * NT;
+    if (i < TOT) P[(i / N1) * PL::PLS + i % N1] = v[p];
   }
   __syncthreads();
 }

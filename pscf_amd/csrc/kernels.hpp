@@ -145,8 +145,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   using PL = Plane<N1, N2>;
   constexpr int NPX = PL::NP * N1;
   constexpr int PER = (NPX + NT - 1) / NT;
-  __shared__ double2 P[PL::PSIZE];
-  __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 U[PL::USIZE];
+  double2* const P = PL::Pbuf(U);
+  double2* const H = PL::Hbuf(U);
   __shared__ double2 TW1[N1], TW2[N2];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
@@ -316,8 +317,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_c(const StepTask* __r
   using PL = Plane<N1, N2>;
   constexpr int NPX = PL::NP * N1;
   constexpr int PER = (NPX + NT - 1) / NT;
-  __shared__ double2 P[PL::PSIZE];
-  __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 U[PL::USIZE];
+  double2* const P = PL::Pbuf(U);
+  double2* const H = PL::Hbuf(U);
   __shared__ double2 TW1[N1], TW2[N2];
   const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
@@ -544,8 +546,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_r2c(const double* __
                                                   const double2* __restrict__ tw2g) {
   constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
-  __shared__ double2 P[PL::PSIZE];
-  __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 U[PL::USIZE];
+  double2* const P = PL::Pbuf(U);
+  double2* const H = PL::Hbuf(U);
   __shared__ double2 tw1[N1], tw2[N2];
   const int tid = threadIdx.x, z = blockIdx.x;
   stage<N1, NT>(tw1, tw1g, tid);
@@ -569,8 +572,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_c2r(const double2* _
                                                   const double2* __restrict__ tw2g) {
   constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
-  __shared__ double2 P[PL::PSIZE];
-  __shared__ double2 H[PL::HSIZE];
+  __shared__ double2 U[PL::USIZE];
+  double2* const P = PL::Pbuf(U);
+  double2* const H = PL::Hbuf(U);
   __shared__ double2 tw1[N1], tw2[N2];
   const int tid = threadIdx.x, z = blockIdx.x;
   stage<N1, NT>(tw1, tw1g, tid);

@@ -3,19 +3,21 @@
 //   max_radix  largest Stockham radix of a transform of this length (8 or 4)
 //   xpad       LDS padding (complex words) per packed row of an XY plane
 //   zsplit     workgroups sharing one ky row of the z-column step kernel
+//   alias      1: the packed-row and half-spectrum LDS buffers of an XY plane
+//              share memory (half the LDS per workgroup, more workgroups per CU)
 #pragma once
 
 namespace pscf {
 
 struct TunedPlan {
-  int n, max_radix, xpad, zsplit;
+  int n, max_radix, xpad, zsplit, alias;
 };
 
 constexpr TunedPlan kTunedPlans[] = {
-    {32, 4, 1, 3},  // 28.63 us/step (radix 8, no pad, 1 row: 31.37)
-    {48, 4, 1, 1},  // 39.78 us/step (radix 8, no pad, 1 row: 52.58)
-    {64, 4, 1, 1},  // 48.74 us/step (radix 8, no pad, 1 row: 55.27)
-    {96, 8, 1, 2},  // 143.22 us/step (radix 8, no pad, 1 row: 157.42)
+    {32, 4, 1, 3, 1},  // 29.15 us/step (radix 8, no pad, 1 row, separate: 32.17)
+    {48, 4, 1, 1, 0},  // 39.88 us/step (radix 8, no pad, 1 row, separate: 52.65)
+    {64, 4, 1, 1, 0},  // 49.07 us/step (radix 8, no pad, 1 row, separate: 55.83)
+    {96, 4, 1, 1, 0},  // 145.36 us/step (radix 8, no pad, 1 row, separate: 163.78)
 };
 
 }  // namespace pscf
