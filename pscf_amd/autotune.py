@@ -1,9 +1,12 @@
 """Autotuner of the contour-step kernel plans (pscf_amd/csrc/tuned.hpp).
 
-The plan of a grid length n is (max_radix, xpad, zsplit, alias): the largest
-Stockham radix of its transforms, the LDS padding of the XY-plane rows, the
-number of workgroups per ky row of the z-column step kernel, and whether the
-packed-row and half-spectrum buffers of an XY plane share LDS.  They are compile-time
+The plan of a grid length n is (max_radix, xpad, zsplit, alias, ajobs): the
+largest Stockham radix of its transforms, the LDS padding of the XY-plane
+rows, the number of workgroups per ky row of the z-column step kernel,
+whether the packed-row and half-spectrum buffers of an XY plane share LDS,
+and the workgroups per plane of k_step_a.  The candidate space below keeps
+the dimensions that mattered in earlier sweeps (row padding always won,
+shared buffers were neutral; both stay in the table format).  They are compile-time
 parameters, so the tuner builds one library per candidate plan (overriding
 the table for every length), times the captured contour sweeps of a
 synthetic system on each candidate and grid length on the device, and writes
@@ -34,18 +37,19 @@ RESULTS = os.path.join(TUNE_DIR, "results.json")
 TUNED_HPP = os.path.join(HERE, "csrc", "tuned.hpp")
 
 RADIX = (8, 4)
-XPAD = (0, 1)
-ZSPLIT = (1, 2, 3)
-ALIAS = (0, 1)
+XPAD = (1,)
+ZSPLIT = (1, 3)
+ALIAS = (0,)
+AJOBS = (2, 1)
 
 
 def candidates():
-    return [dict(max_radix=r, xpad=p, zsplit=z, alias=a)
-            for r, p, z, a in itertools.product(RADIX, XPAD, ZSPLIT, ALIAS)]
+    return [dict(max_radix=r, xpad=p, zsplit=z, alias=a, ajobs=aj)
+            for r, p, z, a, aj in itertools.product(RADIX, XPAD, ZSPLIT, ALIAS, AJOBS)]
 
 
 def variant_name(c):
-    return "r%d_p%d_z%d_a%d" % (c["max_radix"], c["xpad"], c["zsplit"], c["alias"])
+    return "r%d_p%d_z%d_a%d_j%d" % (c["max_radix"], c["xpad"], c["zsplit"], c["alias"], c["ajobs"])
 
 
 def build(jobs=4):
@@ -55,7 +59,8 @@ def build(jobs=4):
     def one(c):
         out = os.path.join(TUNE_DIR, "lib_%s.so" % variant_name(c))
         build_lib(defines={"PSCF_MAX_RADIX": c["max_radix"], "PSCF_XPAD": c["xpad"],
-                           "PSCF_ZSPLIT": c["zsplit"], "PSCF_ALIAS": c["alias"]}, out=out)
+                           "PSCF_ZSPLIT": c["zsplit"], "PSCF_ALIAS": c["alias"],
+                           "PSCF_AJOBS": c["ajobs"]}, out=out)
         return out
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -126,8 +131,8 @@ def write(results=RESULTS):
     rows = []
     for n in sorted(res, key=int):
         best = min(res[n], key=res[n].get)
-        r, p, z, a = [int(t[1:]) for t in best.split("_")]
-        rows.append((int(n), r, p, z, a, res[n][best], res[n].get("r8_p0_z1_a0")))
+        r, p, z, a, aj = [int(t[1:]) for t in best.split("_")]
+        rows.append((int(n), r, p, z, a, aj, res[n][best], res[n].get("r8_p1_z1_a0_j2")))
     lines = [
         "// Kernel plans per FFT/grid length for gfx950, written by pscf_amd/autotune.py",
         "// from timings of the contour sweeps on an MI355X (rerun it to regenerate).",
@@ -136,13 +141,15 @@ def write(results=RESULTS):
         "//   zsplit     workgroups sharing one ky row of the z-column step kernel",
         "//   alias      1: the packed-row and half-spectrum LDS buffers of an XY plane",
         "//              share memory (half the LDS per workgroup, more workgroups per CU)",
+        "//   ajobs      workgroups per plane of k_step_a: 2 (each repeats the inverse",
+        "//              transform and starts one sub-step) or 1 (starts both)",
         "#pragma once", "", "namespace pscf {", "",
-        "struct TunedPlan {", "  int n, max_radix, xpad, zsplit, alias;", "};", "",
+        "struct TunedPlan {", "  int n, max_radix, xpad, zsplit, alias, ajobs;", "};", "",
         "constexpr TunedPlan kTunedPlans[] = {"]
-    for n, r, p, z, a, us, base in rows:
-        note = "  // %.2f us/step (radix 8, no pad, 1 row, separate: %s)" % (
+    for n, r, p, z, a, aj, us, base in rows:
+        note = "  // %.2f us/step (radix 8, pad 1, 1 row, separate, 2 jobs: %s)" % (
             us, "%.2f" % base if base is not None else "n/a")
-        lines.append("    {%d, %d, %d, %d, %d},%s" % (n, r, p, z, a, note))
+        lines.append("    {%d, %d, %d, %d, %d, %d},%s" % (n, r, p, z, a, aj, note))
     lines += ["};", "", "}  // namespace pscf", ""]
     open(TUNED_HPP, "w").write("\n".join(lines))
     print("wrote", TUNED_HPP)

@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   double2* const P = PL::Pbuf(U);
   double2* const H = PL::Hbuf(U);
   __shared__ double2 TW1[N1], TW2[N2];
-  const int tid = threadIdx.x, z = blockIdx.x, job = blockIdx.z;
+  constexpr bool ONEJOB = plan_ajobs(N1) == 1;   // one workgroup per plane starts both A and B
+  const int tid = threadIdx.x, z = blockIdx.x, job = ONEJOB ? 0 : blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
   const int j = jl + t.jofs;
   if (t.skip || j < t.jbeg || j > t.jend || (job == 1 && j == t.jend)) return;
@@ -161,7 +162,9 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   const bool first = j == t.jbeg;
   const double* ef = start ? (job == 0 ? t.e1[task_block(t, j)] : t.e2[task_block(t, j)]) + pbase
                            : nullptr;
+  const double* ef2 = (ONEJOB && start) ? t.e2[task_block(t, j)] + pbase : nullptr;
   double qa[PER], qb[PER], fa[PER], fb[PER], e2a[PER], e2b[PER], q1a[PER], q1b[PER];
+  double ga[ONEJOB ? PER : 1], gb[ONEJOB ? PER : 1];
   // requests first: factors of this step, and for j > 0 the previous step's inputs
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -173,6 +176,10 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
       if (start) {
         fa[p] = ef[o0];
         if (two) fb[p] = ef[o0 + N1];
+        if constexpr (ONEJOB) {
+          ga[p] = ef2[o0];
+          gb[p] = two ? ef2[o0 + N1] : 0.0;
+        }
       }
       if (!first) {
         const double* e2 = t.e2[task_block(t, j - 1)] + pbase;
@@ -300,6 +307,22 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   for (int p = 0; p < (PL::HSIZE + NT - 1) / NT; ++p) {
     const int i = tid + p * NT;
     if (i < PL::HSIZE) out[i] = H[i];
+  }
+  if constexpr (ONEJOB) {          // the half-step start: exp(-ds w/4) q -> B
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int i = tid + p * NT;
+      if (i < NPX) P[PL::pidx(i)] = make_double2(ga[p] * qa[p], gb[p] * qb[p]);
+    }
+    __syncthreads();
+    plane_forward<N1, N2, NT>(P, H, TW1, TW2, tid);
+    double2* outb = t.B + hbase;
+#pragma unroll
+    for (int p = 0; p < (PL::HSIZE + NT - 1) / NT; ++p) {
+      const int i = tid + p * NT;
+      if (i < PL::HSIZE) outb[i] = H[i];
+    }
   }
 }
 

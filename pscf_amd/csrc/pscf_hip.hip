@@ -114,6 +114,7 @@ using C2CFn = void (*)(double2*, long, const double2*, const double2*);
 
 struct PlaneSet {
   int nt = 0, nt_c2c = 0;     // workgroup sizes (plane_nt, c2c_nt)
+  int ajobs = 2;              // workgroups per plane of k_step_a
   StepPlaneFn a = nullptr, c = nullptr;
   R2CFn r2c = nullptr;
   C2RFn c2r = nullptr;
@@ -132,6 +133,7 @@ PlaneSet make_plane() {
   PlaneSet s;
   s.nt = plane_nt(A, B);
   s.nt_c2c = c2c_nt(A, B);
+  s.ajobs = plan_ajobs(A);
   s.a = k_step_a<A, B>;
   s.c = k_step_c<A, B>;
   s.r2c = k_plane_r2c<A, B>;
@@ -457,7 +459,7 @@ void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, h
   };
   for (int j = 0; j <= max_ns; ++j) {
     mark();
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask, 2), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask, g.pl->ajobs), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
                        g.tw1, g.tw2);
     if (j == max_ns) break;
     mark();
@@ -1138,7 +1140,7 @@ void run_adj_sweeps(const System& S, hipStream_t s) {
         run_sweeps(g, A.ck_rec.p + (size_t)r * nt, nt, S.ck.len[r], s);
       tasks = A.ck_adj.p + (size_t)r * nt;
     }
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, 2), dim3(g.pl->nt), 0, s, tasks, j, g.n3, g.tw1,
+    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, g.pl->ajobs), dim3(g.pl->nt), 0, s, tasks, j, g.n3, g.tw1,
                        g.tw2);
     if (j >= 1)
       hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, tasks, A.pairs.p, A.npair,
@@ -1841,7 +1843,7 @@ int pscf_profile_step_kernel(int64_t sys, int kind, int j, int reps, double* us)
     ensure_events(S, 2);
     auto launch = [&] {
       if (kind == 0)
-        hipLaunchKernelGGL(g.pl->a, dim3(g.n3, S.ntask, 2), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
+        hipLaunchKernelGGL(g.pl->a, dim3(g.n3, S.ntask, g.pl->ajobs), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
                            g.tw1, g.tw2);
       else if (kind == 2)
         hipLaunchKernelGGL(g.pl->c, dim3(g.n3, S.ntask, 2), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,

@@ -5,19 +5,21 @@
 //   zsplit     workgroups sharing one ky row of the z-column step kernel
 //   alias      1: the packed-row and half-spectrum LDS buffers of an XY plane
 //              share memory (half the LDS per workgroup, more workgroups per CU)
+//   ajobs      workgroups per plane of k_step_a: 2 (each repeats the inverse
+//              transform and starts one sub-step) or 1 (starts both)
 #pragma once
 
 namespace pscf {
 
 struct TunedPlan {
-  int n, max_radix, xpad, zsplit, alias;
+  int n, max_radix, xpad, zsplit, alias, ajobs;
 };
 
 constexpr TunedPlan kTunedPlans[] = {
-    {32, 4, 1, 3, 1},  // 29.15 us/step (radix 8, no pad, 1 row, separate: 32.17)
-    {48, 4, 1, 1, 0},  // 39.88 us/step (radix 8, no pad, 1 row, separate: 52.65)
-    {64, 4, 1, 1, 0},  // 49.07 us/step (radix 8, no pad, 1 row, separate: 55.83)
-    {96, 4, 1, 1, 0},  // 145.36 us/step (radix 8, no pad, 1 row, separate: 163.78)
+    {32, 4, 1, 3, 1, 2},  // 29.15 us/step (radix 8, no pad, 1 row, separate: 32.17)
+    {48, 4, 1, 1, 0, 2},  // 39.88 us/step (radix 8, no pad, 1 row, separate: 52.65)
+    {64, 4, 1, 1, 0, 2},  // 49.07 us/step (radix 8, no pad, 1 row, separate: 55.83)
+    {96, 4, 1, 1, 0, 2},  // 145.36 us/step (radix 8, no pad, 1 row, separate: 163.78)
 };
 
 }  // namespace pscf

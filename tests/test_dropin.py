@@ -9,7 +9,10 @@ CPU: the GPU library is replaced by its CPU-emulated build (LD_PRELOAD), and
 the first Anderson iterations' residual / free-energy log must agree with the
 reference's.  GPU: both run to convergence and must reproduce the known
 answers that the reference ships (tools/python/tests/outfile/in:39-79,
-lamellar f=0.5, chi N=15).
+lamellar f=0.5, chi N=15).  Both Anderson mixing and the reference's
+Newton-Raphson iteration (iterate_NR with the finite-difference Jacobian of
+response_pd_mod, N_cut = 17 as in that known-answer run) run unchanged on
+top of the replacement density().
 """
 import os
 import re
@@ -38,7 +41,7 @@ def _need(path):
         pytest.skip("%s not built (make -C oracle; make -C pscf_amd/fortran)" % path)
 
 
-def _write_case(tmp_path, max_itr):
+def _write_case(tmp_path, max_itr, algo="AM"):
     # start: a field within ~5% of the lamellar solution (tests/golden/lam32_start.npz),
     # so that both programs take the same, non-chaotic Anderson path
     g = load_golden("lam32")
@@ -46,9 +49,10 @@ def _write_case(tmp_path, max_itr):
     B = scf.Basis.from_tables(g)
     om = load_golden("lam32_start")["omega"]
     fieldio.write_field(str(tmp_path / "in.omega"), om, sysm, B, g["star_count"])
+    tail = "itr_algo\n'AM'\nN_hist\n10\n" if algo == "AM" else "itr_algo\n'NR'\nN_cut\n17\n"
     text = sysm.to_param() + (
         "ITERATE\ninput_filename\n'in.omega'\noutput_prefix\n'out/'\nmax_itr\n%d\n"
-        "error_max\n1.0E-10\ndomain\nF\nitr_algo\n'AM'\nN_hist\n10\n\nFINISH\n" % max_itr)
+        "error_max\n1.0E-10\ndomain\nF\n%s\nFINISH\n" % (max_itr, tail))
     (tmp_path / "param").write_text(text)
     (tmp_path / "out").mkdir(exist_ok=True)
 
@@ -92,12 +96,32 @@ def test_dropin_trajectory_matches_reference_cpu(tmp_path):
         assert abs(f2 / f - 1) < 1e-9, (i, f, f2)
 
 
+def test_dropin_newton_trajectory_matches_reference_cpu(tmp_path):
+    """iterate_NR: each iteration builds the Jacobian from N_cut * N_monomer
+    perturbed density() calls (increment 1e-8, src/response_pd_mod.f90:91-147),
+    so residuals and free energies after the first step agree only to the FD
+    rounding amplification (~1e-4 and ~1e-8 relative)."""
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("emu")
+    _write_case(tmp_path, max_itr=2, algo="NR")
+    ref = _log(_run(REF, str(tmp_path)))
+    env = dict(os.environ, LD_PRELOAD=EMU_LIB)
+    mine = _log(_run(GPU_BIN, str(tmp_path), env=env))
+    assert len(ref) >= 2 and len(mine) == len(ref)
+    for (i, e, f), (j, e2, f2) in zip(ref, mine):
+        assert i == j
+        assert abs(e2 / e - 1) < 1e-3, (i, e, e2)
+        assert abs(f2 / f - 1) < 1e-7, (i, f, f2)
+
+
 @pytest.mark.gpu
-def test_dropin_converges_to_known_answer_gpu(tmp_path):
+@pytest.mark.parametrize("algo", ["AM", "NR"])
+def test_dropin_converges_to_known_answer_gpu(tmp_path, algo):
     _need(REF)
     _need(GPU_BIN)
     get_lib("gpu")
-    _write_case(tmp_path, max_itr=300)
+    _write_case(tmp_path, max_itr=300, algo=algo)
     ref_dir = tmp_path / "ref"
     gpu_dir = tmp_path / "gpu"
     for d in (ref_dir, gpu_dir):
