@@ -16,10 +16,10 @@ struct TunedPlan {
 };
 
 constexpr TunedPlan kTunedPlans[] = {
-    {32, 4, 1, 3, 1, 2},  // 29.15 us/step (radix 8, no pad, 1 row, separate: 32.17)
-    {48, 4, 1, 1, 0, 2},  // 39.88 us/step (radix 8, no pad, 1 row, separate: 52.65)
-    {64, 4, 1, 1, 0, 2},  // 49.07 us/step (radix 8, no pad, 1 row, separate: 55.83)
-    {96, 4, 1, 1, 0, 2},  // 145.36 us/step (radix 8, no pad, 1 row, separate: 163.78)
+    {32, 4, 1, 3, 0, 2},  // 29.13 us/step (radix 8, pad 1, 1 row, separate, 2 jobs: 31.60)
+    {48, 4, 1, 1, 0, 2},  // 39.63 us/step (radix 8, pad 1, 1 row, separate, 2 jobs: 52.01)
+    {64, 4, 1, 1, 0, 2},  // 48.92 us/step (radix 8, pad 1, 1 row, separate, 2 jobs: 51.95)
+    {96, 4, 1, 1, 0, 1},  // 129.25 us/step (radix 8, pad 1, 1 row, separate, 2 jobs: 166.22)
 };
 
 }  // namespace pscf
