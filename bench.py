@@ -258,6 +258,44 @@ def traffic_probe():
     print("traffic probe done", flush=True)
 
 
+def task_scaling(L, counts=(1, 2, 4, 8, 16)):
+    """Contour-step cost versus sweeps per launch: one BCC 64^3 system whose
+    chemistry holds n identical AB diblock species (N=100, f=0.25, ds=1), so
+    every launch carries 2n sweeps.  Prints per-chain step and kernel times."""
+    import torch
+    from pscf_amd import iterate, scf
+    from pscf_amd.system import System
+    d = np.load(os.path.join(ROOT, "tests", "golden", "bcc64.npz"))
+    g = {k: d[k] for k in d.files}
+    B = scf.Basis.from_tables(g)
+    _, G, _ = iterate.unit_cell(L, 3, "cubic", [18.0])
+    out = []
+    for n in counts:
+        chi = 0.2
+        s = System(kuhn=[1.0, 1.0], chains=[[(1, 25.0), (2, 75.0)]] * n, phi_chain=[1.0 / n] * n,
+                   chi=np.array([[0, chi], [chi, 0]]), dim=3, crystal_system="cubic",
+                   cell_param=[18.0], ngrid=[64, 64, 64], chain_step=1.0, group_name="I m -3 m")
+        S = scf.SCFSystem(s, B, G, lib=L)
+        S.density(g["omega"])
+        S.profile_sweeps(1)
+        for _ in range(3):
+            S.density(g["omega"])
+        ms, cnt = S.profile_sweeps(0)
+        step = float(ms[4]) / max(int(cnt[4]), 1) * 1e3
+        ks = []
+        for kind in range(4):
+            S.density(g["omega"])
+            ks.append(S.profile_step_kernel(kind, 50, reps=50))
+        torch.cuda.synchronize()
+        row = {"chains": n, "sweeps_per_launch": 2 * n, "step_us": step, "step_us_per_chain": step / n,
+               "kernel_us_per_chain": dict(zip(["step_a", "step_z0", "step_c", "step_z1"],
+                                               [k / n for k in ks]))}
+        print(json.dumps(row), flush=True)
+        out.append(row)
+        del S
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,12 +304,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient-cost leg")
+    ap.add_argument("--task-scaling", action="store_true",
+                    help="only print contour-step cost versus sweeps per launch (BCC 64^3)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run two eagerly launched density() calls of the workload and a "
                          "256 MiB device copy (the HBM counter runs under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.task_scaling:
+        from pscf_amd._lib import lib as _lib
+        task_scaling(_lib())
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -4,7 +4,8 @@ The plan of a grid length n is (max_radix, xpad, zsplit, alias, ajobs): the
 largest Stockham radix of its transforms, the LDS padding of the XY-plane
 rows, the number of workgroups per ky row of the z-column step kernel,
 whether the packed-row and half-spectrum buffers of an XY plane share LDS,
-and the workgroups per plane of k_step_a.  The candidate space below keeps
+and the workgroups per plane of k_step_a (0: chosen at launch from the
+number of planes, CUs and resident workgroups).  The candidate space below keeps
 the dimensions that mattered in earlier sweeps (row padding always won,
 shared buffers were neutral; both stay in the table format).  They are compile-time
 parameters, so the tuner builds one library per candidate plan (overriding
@@ -40,7 +41,7 @@ RADIX = (8, 4)
 XPAD = (1,)
 ZSPLIT = (1, 3)
 ALIAS = (0,)
-AJOBS = (2, 1)
+AJOBS = (0, 2, 1)
 
 
 def candidates():
@@ -142,7 +143,9 @@ def write(results=RESULTS):
         "//   alias      1: the packed-row and half-spectrum LDS buffers of an XY plane",
         "//              share memory (half the LDS per workgroup, more workgroups per CU)",
         "//   ajobs      workgroups per plane of k_step_a: 2 (each repeats the inverse",
-        "//              transform and starts one sub-step) or 1 (starts both)",
+        "//              transform and starts one sub-step), 1 (starts both), or 0: 2 while",
+        "//              one per plane would leave CUs idle and two fit in one round of",
+        "//              resident workgroups, else 1",
         "#pragma once", "", "namespace pscf {", "",
         "struct TunedPlan {", "  int n, max_radix, xpad, zsplit, alias, ajobs;", "};", "",
         "constexpr TunedPlan kTunedPlans[] = {"]

@@ -33,13 +33,13 @@ template <int S> __device__ __forceinline__ double2 cmul_si(double2 a) {
 // ------------------------------------------------------ compile-time factoring
 // Per-length kernel plan: the tuned table (tuned.hpp) or, for lengths it does
 // not list, radix 8, no padding, one workgroup per z row, separate plane
-// buffers, two workgroups per plane in k_step_a.  PSCF_MAX_RADIX, PSCF_XPAD,
+// buffers, k_step_a's workgroups per plane chosen at launch.  PSCF_MAX_RADIX, PSCF_XPAD,
 // PSCF_ZSPLIT, PSCF_ALIAS and PSCF_AJOBS, when defined, override the plan for
 // every length (the variants the autotuner builds and times).
 constexpr TunedPlan plan_for(int n) {
   for (const TunedPlan& p : kTunedPlans)
     if (p.n == n) return p;
-  return TunedPlan{n, 8, 0, 1, 0, 2};
+  return TunedPlan{n, 8, 0, 1, 0, 0};
 }
 constexpr int plan_ajobs(int n) {
 #ifdef PSCF_AJOBS

@@ -129,7 +129,8 @@ __device__ __forceinline__ void stage(double2* dst, const double2* __restrict__ 
 }
 
 // ----------------------------------------------------------------------------
-// Step kernel A (two workgroups per z plane and sweep).  Both finish step j-1:
+// Step kernel A (ONEJOB false: two workgroups per z plane and sweep; true: one
+// workgroup does both jobs, one inverse transform fewer).  Both finish step j-1:
 // x,y inverse transform of the second half step's spectrum C, times
 // exp(-ds w/4)/M, Richardson q = (4 q2 - q1)/3 (job 0 stores it to the slab).
 // Then step j starts: job 0 transforms exp(-ds w/2) q into A, job 1
@@ -137,7 +138,7 @@ __device__ __forceinline__ void stage(double2* dst, const double2* __restrict__ 
 // Every global input of the thread is requested at kernel entry, and the
 // twiddles are staged in LDS, so the transform passes wait on LDS only.
 // ----------------------------------------------------------------------------
-template <int N1, int N2>
+template <int N1, int N2, bool ONEJOB>
 __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __restrict__ tasks, int jl, int N3,
                                                const double2* __restrict__ tw1g,
                                                const double2* __restrict__ tw2g) {
@@ -149,7 +150,6 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_step_a(const StepTask* __r
   double2* const P = PL::Pbuf(U);
   double2* const H = PL::Hbuf(U);
   __shared__ double2 TW1[N1], TW2[N2];
-  constexpr bool ONEJOB = plan_ajobs(N1) == 1;   // one workgroup per plane starts both A and B
   const int tid = threadIdx.x, z = blockIdx.x, job = ONEJOB ? 0 : blockIdx.z;
   const StepTask& t = tasks[blockIdx.y];
   const int j = jl + t.jofs;

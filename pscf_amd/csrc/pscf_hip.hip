@@ -114,7 +114,9 @@ using C2CFn = void (*)(double2*, long, const double2*, const double2*);
 
 struct PlaneSet {
   int nt = 0, nt_c2c = 0;     // workgroup sizes (plane_nt, c2c_nt)
-  int ajobs = 2;              // workgroups per plane of k_step_a
+  int ajobs = 0;              // workgroups per plane of k_step_a (0: by occupancy)
+  size_t lds = 0;             // LDS bytes of a plane workgroup
+  StepPlaneFn a1 = nullptr;   // k_step_a, one workgroup per plane
   StepPlaneFn a = nullptr, c = nullptr;
   R2CFn r2c = nullptr;
   C2RFn c2r = nullptr;
@@ -134,7 +136,9 @@ PlaneSet make_plane() {
   s.nt = plane_nt(A, B);
   s.nt_c2c = c2c_nt(A, B);
   s.ajobs = plan_ajobs(A);
-  s.a = k_step_a<A, B>;
+  s.lds = sizeof(double2) * (Plane<A, B>::USIZE + A + B);
+  s.a = k_step_a<A, B, false>;
+  s.a1 = k_step_a<A, B, true>;
   s.c = k_step_c<A, B>;
   s.r2c = k_plane_r2c<A, B>;
   s.c2r = k_plane_c2r<A, B>;
@@ -199,6 +203,8 @@ struct Grid {
   long m = 0, mk = 0;
   const PlaneSet* pl = nullptr;
   const ZSet* zs = nullptr;
+  long cus = 0;               // compute units
+  long plane_slots = 0;       // plane workgroups resident at once (CUs x LDS fit)
   const double2 *tw1 = nullptr, *tw2 = nullptr, *tw3 = nullptr;
 
   void init(const int ng[3], int d) {
@@ -218,6 +224,12 @@ struct Grid {
     REQUIRE(ip != plane_table().end() && iz != z_table().end(), buf);
     pl = &ip->second;
     zs = &iz->second;
+    int dev = 0, cus = 0, lds_cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    this->cus = cus;
+    plane_slots = (long)cus * std::max<long>(1, (long)lds_cu / (long)pl->lds);
     tw1 = twiddles(n1);
     tw2 = twiddles(n2);
     tw3 = twiddles(n3);
@@ -447,6 +459,17 @@ void dev_sum(const double* x, long n, double scale, double* part, double* out_de
   hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(NTA), 0, s, part, nb, scale, out_dev);
 }
 
+// k_step_a for ntask sweeps: two workgroups per plane (more parallelism)
+// while one per plane would leave CUs idle and two all fit in one round of
+// resident workgroups; else one (a quarter less work).  A plan may force either.
+void launch_step_a(const Grid& g, const StepTask* tasks, int ntask, int j, hipStream_t s) {
+  int aj = g.pl->ajobs;
+  const long planes = (long)g.n3 * ntask;
+  if (aj == 0) aj = (planes < g.cus && 2 * planes <= g.plane_slots) ? 2 : 1;
+  hipLaunchKernelGGL(aj == 1 ? g.pl->a1 : g.pl->a, dim3(g.n3, ntask, aj), dim3(g.pl->nt), 0, s, tasks,
+                     j, g.n3, g.tw1, g.tw2);
+}
+
 // --------------------------------------------------- contour sweeps
 // Advance the given sweep tasks through all their steps (4 launches per step).
 // With `ev` (4*max_ns + 2 events) an event is recorded before every launch and
@@ -459,8 +482,7 @@ void run_sweeps(const Grid& g, const StepTask* d_tasks, int ntask, int max_ns, h
   };
   for (int j = 0; j <= max_ns; ++j) {
     mark();
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, ntask, g.pl->ajobs), dim3(g.pl->nt), 0, s, d_tasks, j, g.n3,
-                       g.tw1, g.tw2);
+    launch_step_a(g, d_tasks, ntask, j, s);
     if (j == max_ns) break;
     mark();
     hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, ntask, 2), dim3(g.zs->nt_step), 0, s, d_tasks, j, g.n2,
@@ -1140,8 +1162,7 @@ void run_adj_sweeps(const System& S, hipStream_t s) {
         run_sweeps(g, A.ck_rec.p + (size_t)r * nt, nt, S.ck.len[r], s);
       tasks = A.ck_adj.p + (size_t)r * nt;
     }
-    hipLaunchKernelGGL(g.pl->a, dim3(g.n3, nt, g.pl->ajobs), dim3(g.pl->nt), 0, s, tasks, j, g.n3, g.tw1,
-                       g.tw2);
+    launch_step_a(g, tasks, nt, j, s);
     if (j >= 1)
       hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, tasks, A.pairs.p, A.npair,
                          j - 1, g.m);
@@ -1843,8 +1864,7 @@ int pscf_profile_step_kernel(int64_t sys, int kind, int j, int reps, double* us)
     ensure_events(S, 2);
     auto launch = [&] {
       if (kind == 0)
-        hipLaunchKernelGGL(g.pl->a, dim3(g.n3, S.ntask, g.pl->ajobs), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
-                           g.tw1, g.tw2);
+        launch_step_a(g, S.tasks.p, S.ntask, j, s);
       else if (kind == 2)
         hipLaunchKernelGGL(g.pl->c, dim3(g.n3, S.ntask, 2), dim3(g.pl->nt), 0, s, S.tasks.p, j, g.n3,
                            g.tw1, g.tw2);

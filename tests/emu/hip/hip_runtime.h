@@ -44,6 +44,15 @@ enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDevi
 inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated HIP error" : "no error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor };
+// an MI355X-like device: 256 CUs of 160 KiB LDS (PSCF_EMU_CUS overrides the
+// CU count, so tests can steer launch-shape choices that depend on it)
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+  const char* e = std::getenv("PSCF_EMU_CUS");
+  *v = a == hipDeviceAttributeMultiprocessorCount ? (e ? std::atoi(e) : 256) : 160 * 1024;
+  return hipSuccess;
+}
 inline hipError_t hipMalloc(void** p, size_t n) {
   *p = std::malloc(n ? n : 1);
   if (*p) std::memset(*p, 0xff, n);   // poison: uninitialised reads show up as NaN

@@ -230,6 +230,20 @@ def test_checkpointed_storage_matches_full(backend, name, step, stride):
         S2.scf_stress(np.zeros((S2.N_star, 1)))
 
 
+def test_one_workgroup_step_kernel_adjoint_emu(monkeypatch):
+    """The adjoint and checkpointed sweeps with k_step_a's one-workgroup form
+    (forced by an emulated device reporting no CUs) give the gradients of the
+    two-workgroup form."""
+    S, s, g, omg, dev = _make("blend_nosolv", "emu", 0.09)
+    ref = _loss_and_grads(S, s, omg, dev)
+    monkeypatch.setenv("PSCF_EMU_CUS", "0")
+    S1, s1, g1, omg1, dev1 = _make("blend_nosolv", "emu", 0.09)
+    S1.set_checkpoint(3)
+    out = _loss_and_grads(S1, s1, omg1, dev1)
+    for nm, a, b in zip(["rho", "Q", "grad_omega", "grad_cell", "grad_length"], out, ref):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max(), nm
+
+
 @pytest.mark.gpu
 def test_gyroid64_checkpoint_stride14_gpu():
     """The gyroid 64^3 gradient with q checkpointed every 14 of the 200 contour

@@ -197,3 +197,69 @@ def test_contour_slabs_match_oracle(backend):
     for s in range(ch.qf.shape[0]):
         assert rel(S.q(0, +1, s), ch.qf[s]) < 1e-12, s
         assert rel(S.q(0, -1, s), ch.qr[s]) < 1e-12, s
+
+
+def _grid_density_vs_oracle(backend, n, steps_ds):
+    """density() on an n^3 grid (every compiled plane/z kernel plan of that
+    length) from a seeded random omega grid, against the oracle's
+    chain_density: AB diblock N=10, f=0.3, cubic cell a = n/4."""
+    import ctypes
+    import torch
+    from pscf_amd import batch
+    from pscf_amd.system import diblock
+    lib = get_lib(backend)
+    s = diblock(0.3, 10.0, 15.0, dim=3, crystal_system="cubic", cell_param=[n / 4.0],
+                ngrid=[n, n, n], chain_step=steps_ds, group_name="P 1")
+    G = np.eye(3) * (2.0 * np.pi / (n / 4.0))
+    basis = scf.Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
+    S = scf.SCFSystem(s, basis, G, lib=lib)
+    rng = np.random.default_rng(n)
+    om = 1.5 + 0.3 * rng.standard_normal((2, n, n, n))
+    dev = lib.tensor_device
+    omt = torch.tensor(om, device=dev)
+    rho = torch.empty_like(omt)
+    q = np.zeros(1)
+    qs = np.zeros(1)
+    from pscf_amd._lib import dptr
+    if dev == "cuda":
+        torch.cuda.synchronize()
+    lib.call("pscf_density_grid_dev", S.h, ctypes.c_void_p(omt.data_ptr()),
+             ctypes.c_void_p(rho.data_ptr()), dptr(q), dptr(qs), dptr(S.phi_chain),
+             dptr(S.mu_chain), dptr(S.phi_solvent), dptr(S.mu_solvent))
+    chem = O.Chemistry(s.kuhn, s.chains, s.phi_chain, s.chi)
+    bgn, bds = O.chain_grid([L for _, L in s.chains[0]], steps_ds)
+    ch = O.chain_density(0, O.ChainState(bgn, bds), chem, O.make_ksq([n, n, n], G), om,
+                         O.Propagator([n, n, n]))
+    ref = np.zeros_like(om)
+    for ib, (mon, _) in enumerate(s.chains[0]):
+        ref[mon - 1] += s.phi_chain[0] * ch.rho[ib]
+    assert abs(q[0] / ch.bigQ - 1) < 1e-12, (q[0], ch.bigQ)
+    assert rel(rho.cpu().numpy(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_grid_density_small_sizes(backend, n):
+    _grid_density_vs_oracle(backend, n, 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [40, 96])
+def test_grid_density_large_sizes_gpu(n):
+    """40^3 (radix-5 factor) and 96^3 (the plan with one k_step_a workgroup
+    per plane and radix-3 factors)."""
+    _grid_density_vs_oracle("gpu", n, 2.5)
+
+
+def test_one_workgroup_step_kernel_emu(monkeypatch):
+    """k_step_a takes one workgroup per plane when its two-workgroup grid
+    exceeds one round of resident workgroups.  An emulated device reporting no
+    CUs makes every launch take that form: the reference's density and stress
+    still hold (blend12: 3 + 1 blocks, two chains, solvent)."""
+    monkeypatch.setenv("PSCF_EMU_CUS", "0")
+    lib = get_lib("emu")
+    for name in ("bcc8", "blend12"):
+        g, sysm, S = system_for(name, lib)
+        rho, q, qs = S.density(g["omega"])
+        assert rel(q, g["Q"]) < 1e-11, name
+        assert rel(rho, g["rho"]) < 1e-10, name
+        assert rel(S.scf_stress(g["dGsq"]), g["stress"]) < 1e-9, name
