@@ -318,8 +318,8 @@ def main():
         task_scaling(_lib())
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(independent solves shard across GPUs without communication; see DESIGN.md)")
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(independent solves would shard across GPUs without communication)")
         sys.exit(2)
 
     import torch
