@@ -300,7 +300,8 @@ struct ChainDev {
   std::vector<int> slot;
   int nslot = 0;
   DevBuf<int> dslot;
-  DevBuf<double> seg;      // recomputed segments: 4 x (K+1) points (f, r; two sides)
+  DevBuf<double> seg;      // recomputed segments: 4 x segpts points (f, r; two sides)
+  int segpts = 0;          // points per segment buffer: min(K, ns) + 1
   DevBuf<double> blk;      // per block: Simpson integral of the last density() (m each)
   const double* point(int dir, int sp) const {    // stored contour point, or null
     const int sl = slot.empty() ? sp : slot[sp];
@@ -668,7 +669,7 @@ void build_ck_plan(System& S) {
         StepTask t = S.h_tasks[2 * c + d];
         StepTask& o = ht[((size_t)r * nc + c) * 2 + d];
         t.init_one = 0;
-        t.slab = ch.seg.p + (size_t)d * (K + 1) * S.g.m;
+        t.slab = ch.seg.p + (size_t)d * ch.segpts * S.g.m;
         t.pmap = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 0];
         if (lo >= hi) {
           t.skip = 1;
@@ -711,7 +712,7 @@ void ck_simpson_rounds(System& S, hipStream_t s) {
       for (int b = 0; b <= ch.nblk; ++b) a.bnd[b] = ch.bnd[b];
       for (int b = 0; b < ch.nblk; ++b) a.ds[b] = ch.ds[b];
       hipLaunchKernelGGL(k_simpson_seg, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.seg.p,
-                         ch.seg.p + (size_t)(K + 1) * g.m, g.m, a, lo, hi, hi == ch.ns ? 1 : 0,
+                         ch.seg.p + (size_t)ch.segpts * g.m, g.m, a, lo, hi, hi == ch.ns ? 1 : 0,
                          ch.blk.p);
     }
   }
@@ -746,7 +747,8 @@ void startup(System& S, double chain_step) {
       for (int sp = 0; sp <= ch->ns; ++sp)
         if (sp % K == 0 || (ch->ns - sp) % K == 0) ch->slot[sp] = ch->nslot++;
       ch->dslot.upload(ch->slot.data(), ch->slot.size(), S.st.s);
-      ch->seg.alloc((size_t)4 * (K + 1) * g.m);
+      ch->segpts = std::min(K, ch->ns) + 1;
+      ch->seg.alloc((size_t)4 * ch->segpts * g.m);
       ch->blk.alloc((size_t)ch->nblk * g.m);
     } else {
       ch->nslot = ch->ns + 1;
@@ -1092,7 +1094,7 @@ void adjoint_setup(System& S, bool with_k) {
     for (int r = 0; r < P.nround; ++r)
       for (int c = 0; c < nc; ++c) {
         ChainDev& ch = *S.chains[c];
-        auto seg = [&](int i) { return ch.seg.p + (size_t)i * (K + 1) * g.m; };
+        auto seg = [&](int i) { return ch.seg.p + (size_t)i * ch.segpts * g.m; };
         const int* mapR = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 0];
         const int* mapF = P.maps.p + P.map_off[((size_t)r * nc + c) * 2 + 1];
         // adjoint tasks of this round; a chain whose steps ended in an earlier

@@ -188,6 +188,15 @@ def test_identities_converge_with_ds(backend, which):
     assert gaps[1] < (5e-4 if which == "omega" else 1e-3), gaps
 
 
+def test_adjoint_rejects_solvents(backend):
+    """Systems with solvent species have no adjoint: backward fails loudly."""
+    S, s, g, omg, dev = _make("blend12", backend)
+    om = omg.clone().requires_grad_(True)
+    rho, Q = adjoint.density(S, om)
+    with pytest.raises(Exception, match="solvent"):
+        rho.sum().backward()
+
+
 def test_backward_after_new_forward_raises(backend):
     S, s, g, omg, dev = _make("bcc8", backend)
     om = omg.clone().requires_grad_(True)
@@ -212,7 +221,8 @@ def _loss_and_grads(S, s, omg, dev, seed=11):
     return [x.detach().cpu().numpy() for x in (rho, Q, om.grad, cp.grad, Lb.grad)]
 
 
-@pytest.mark.parametrize("name,step,stride", [("bcc8", None, 3), ("blend_nosolv", 0.09, 3)])
+@pytest.mark.parametrize("name,step,stride", [("bcc8", None, 3), ("bcc8", None, 1000),
+                                              ("blend_nosolv", 0.09, 3)])
 def test_checkpointed_storage_matches_full(backend, name, step, stride):
     """q and q-dagger kept only every `stride` contour points (and at the
     points counted from Ns), the rest recomputed per segment: density and every
