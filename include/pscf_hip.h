@@ -257,11 +257,12 @@ int pscf_density_grid_dev(int64_t sys, const double* omega_grid, double* rho_gri
  * NULL).  Out: grad_omega_grid = dL/domega_grid; with dGG (dim, dim,
  * n_cell_param) of make_unit_cell (host; may be NULL) dL_dcell(n_cell_param);
  * with dL_dlength (host; may be NULL) dL/d block length for every block of
- * every chain, chain-major, at fixed block step counts.  Systems with
- * solvent species are rejected. */
+ * every chain, chain-major, at fixed block step counts; with dL_dphi (host;
+ * may be NULL) dL/d phi_chain(N_chain) at fixed omega (canonical ensemble).
+ * Systems with solvent species are rejected. */
 int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const double* dL_dQ,
                              double* grad_omega_grid, int n_cell_param, const double* dGG,
-                             double* dL_dcell, double* dL_dlength);
+                             double* dL_dcell, double* dL_dlength, double* dL_dphi);
 /* basis_to_kgrid + ifft of nf star fields, device basis (nf, N_star) ->
  * device grids (M, nf)                        src/scf_mod.fp.f90:182-185 */
 int pscf_basis_to_grid_dev(int64_t sys, const double* basis, int nf, double* grid);

@@ -54,7 +54,7 @@ SIGNATURES = {
     "pscf_density_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
                               _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pscf_density_adjoint_dev": [ctypes.c_int64, ctypes.c_void_p, _c_dbl_p, ctypes.c_void_p,
-                                 ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+                                 ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pscf_basis_to_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "pscf_grid_to_basis_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                ctypes.c_void_p],

@@ -78,11 +78,12 @@ def set_state(S, cell_param=None, lengths=None):
 
 
 class DensityFunction(torch.autograd.Function):
-    """(omega_grid, cell_param, block_lengths) -> (rho_grid, Q)."""
+    """(omega_grid, cell_param, block_lengths, phi_chain) -> (rho_grid, Q)."""
 
     @staticmethod
-    def forward(ctx, omega_grid, cell_param, lengths, S):
+    def forward(ctx, omega_grid, cell_param, lengths, phi, S):
         set_state(S, cell_param.detach().cpu().numpy(), lengths.detach().cpu().numpy())
+        S.phi_chain[: len(phi)] = phi.detach().cpu().numpy()
         s = S.system
         om = omega_grid.detach().to(torch.float64).contiguous()
         assert om.shape == (s.N_monomer,) + S.shape, (om.shape, S.shape)
@@ -111,10 +112,11 @@ class DensityFunction(torch.autograd.Function):
         dQ = (np.zeros(max(s.N_chain, 1)) if g_Q is None
               else np.ascontiguousarray(g_Q.detach().cpu().numpy(), dtype=np.float64))
         grad = torch.empty_like(lam)
-        want_cell, want_len = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+        want_cell, want_len, want_phi = ctx.needs_input_grad[1:4]
         ncp = ctx.ncp
         dcell = np.zeros(max(ncp, 1))
         dlen = np.zeros(sum(len(c) for c in s.chains))
+        dphi = np.zeros(max(s.N_chain, 1))
         dgg_arg = None
         if want_cell:
             _, _, dGG = unit_cell(S.lib, s.dim, s.crystal_system, s.cell_param)
@@ -122,25 +124,28 @@ class DensityFunction(torch.autograd.Function):
         _sync(lam)
         S.lib.call("pscf_density_adjoint_dev", S.h, _vp(lam), dptr(dQ), _vp(grad), ncp,
                    dptr(dgg_arg) if want_cell else None, dptr(dcell) if want_cell else None,
-                   dptr(dlen) if want_len else None)
+                   dptr(dlen) if want_len else None, dptr(dphi) if want_phi else None)
         return (grad,
                 torch.tensor(dcell[:ncp], dtype=torch.float64) if want_cell else None,
                 torch.tensor(dlen, dtype=torch.float64) if want_len else None,
+                torch.tensor(dphi[: s.N_chain], dtype=torch.float64) if want_phi else None,
                 None)
 
 
-def density(S, omega_grid, cell_param=None, lengths=None):
-    """rho_grid, Q of system S (an SCFSystem) at omega_grid; cell_param and
-    lengths (float64 tensors, optional) make the cell and block lengths
-    differentiable inputs."""
+def density(S, omega_grid, cell_param=None, lengths=None, phi=None):
+    """rho_grid, Q of system S (an SCFSystem) at omega_grid; cell_param,
+    lengths (block lengths, chain-major) and phi (chain volume fractions),
+    float64 tensors, optional, make those differentiable inputs."""
     if cell_param is None:
         cell_param = torch.tensor(np.asarray(S.system.cell_param, dtype=np.float64))
     if lengths is None:
         lengths = block_lengths(S.system)
-    return DensityFunction.apply(omega_grid, cell_param, lengths, S)
+    if phi is None:
+        phi = torch.tensor(np.asarray(S.phi_chain[: S.system.N_chain], dtype=np.float64))
+    return DensityFunction.apply(omega_grid, cell_param, lengths, phi, S)
 
 
-def free_energy(S, omega_grid, rho_grid, Q, lengths=None):
+def free_energy(S, omega_grid, rho_grid, Q, lengths=None, phi=None):
     """Helmholtz free energy per monomer (src/scf_mod.fp.f90:834-886), canonical
     ensemble, with the star sums written as grid averages.  The two agree
     except for stars with waves on a Nyquist plane, whose grid functions are
@@ -157,10 +162,10 @@ def free_energy(S, omega_grid, rho_grid, Q, lengths=None):
     for c, ch in enumerate(s.chains):
         N = lengths[k:k + len(ch)].sum().to(dev)
         k += len(ch)
-        phi = float(s.phi_chain[c])
-        if phi > 1.0e-8:
-            mu = torch.log(phi / Q[c])
-            f = f + phi * (mu - 1.0) / N
+        ph = phi[c].to(dev) if phi is not None else float(s.phi_chain[c])
+        if float(ph) > 1.0e-8:
+            mu = torch.log(ph / Q[c])
+            f = f + ph * (mu - 1.0) / N
     if len(s.solvents):
         raise ValueError("free_energy: solvent species are not supported here")
     nm = s.N_monomer
@@ -173,7 +178,7 @@ def free_energy(S, omega_grid, rho_grid, Q, lengths=None):
     return f + inter.mean()
 
 
-def free_energy_stars(S, omega_stars, rho_stars, Q, lengths=None):
+def free_energy_stars(S, omega_stars, rho_stars, Q, lengths=None, phi=None):
     """free_energy (src/scf_mod.fp.f90:834-886) in star coefficients, term for
     term as the reference sums it: (N_monomer, N_star) omega and rho."""
     s = S.system
@@ -186,9 +191,9 @@ def free_energy_stars(S, omega_stars, rho_stars, Q, lengths=None):
     for c, ch in enumerate(s.chains):
         N = lengths[k:k + len(ch)].sum().to(rho_stars.device)
         k += len(ch)
-        phi = float(s.phi_chain[c])
-        if phi > 1.0e-8:
-            f = f + phi * (torch.log(phi / Q[c]) - 1.0) / N
+        ph = phi[c].to(rho_stars.device) if phi is not None else float(s.phi_chain[c])
+        if float(ph) > 1.0e-8:
+            f = f + ph * (torch.log(ph / Q[c]) - 1.0) / N
     chi = np.asarray(s.chi, dtype=np.float64)
     nm = s.N_monomer
     tot = torch.zeros_like(rho_stars[0])

@@ -1196,9 +1196,10 @@ double host_sum(System& S, const double* part, int np) {
 // (M, N_monomer, device) = dL/d omega_grid; with dgg (dim, dim, ncp) also
 // dL/d cell_param (ncp); with dlen also dL/d block length per chain and block
 // (block step count fixed; the chain length in the prefactor is the sum of
-// the block lengths).
+// the block lengths); with dphi dL/d phi_chain (canonical ensemble: rho of a
+// chain is proportional to its phi; 0 in the grand ensemble).
 void density_adjoint(System& S, const double* lam, const double* dQ, double* grad, int ncp,
-                     const double* dgg, double* dcell, double* dlen) {
+                     const double* dgg, double* dcell, double* dlen, double* dphi) {
   REQUIRE(S.have_density, "the adjoint requires a previous density()");
   REQUIRE(S.nsolv == 0, "the adjoint does not support solvent species");
   const bool with_k = dgg != nullptr || dlen != nullptr;
@@ -1299,6 +1300,14 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
       hipLaunchKernelGGL(k_dksq_dot, dim3(nb, ncp), dim3(NTA), 0, s, ga, d_dgg.p,
                          A.kblk.p + k * g.mk, part);
       for (int p = 0; p < ncp; ++p) dcell[p] += host_sum(S, part + (size_t)p * nb, nb);
+    }
+  }
+  if (dphi) {
+    for (int c = 0; c < S.nchain; ++c) {
+      double tsum = 0.0;
+      for (double t : T[c]) tsum += t;
+      const double phi = S.h_chem[2 * c];
+      dphi[c] = (S.ensemble == 0 && phi != 0.0) ? coef[c] * tsum / phi : 0.0;
     }
   }
   if (dlen) {
@@ -2145,12 +2154,12 @@ int pscf_density_grid_dev(int64_t sys, const double* omega_grid, double* rho_gri
 
 int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const double* dL_dQ,
                              double* grad_omega_grid, int n_cell_param, const double* dGG,
-                             double* dL_dcell, double* dL_dlength) {
+                             double* dL_dcell, double* dL_dlength, double* dL_dphi) {
   return guarded([&] {
     REQUIRE(lam_rho_grid && grad_omega_grid, "null grid pointer");
     REQUIRE(!dGG || dL_dcell, "dGG given without an output for the cell gradient");
     density_adjoint(*from_handle<System>(sys, "system"), lam_rho_grid, dL_dQ, grad_omega_grid,
-                    n_cell_param, dGG, dL_dcell, dL_dlength);
+                    n_cell_param, dGG, dL_dcell, dL_dlength, dL_dphi);
   });
 }
 

@@ -71,8 +71,8 @@ def test_gradients_match_central_fd(backend, name, step):
     rng = np.random.default_rng(11)
     tgt = torch.tensor(rng.normal(0.3, 0.05, omg.shape[1:]), device=dev)
 
-    def loss(om, cp, L):
-        rho, Q = adjoint.density(S, om, cp, L)
+    def loss(om, cp, L, ph):
+        rho, Q = adjoint.density(S, om, cp, L, ph)
         out = ((rho[0] - tgt) ** 2).sum()
         for c in range(s.N_chain):
             out = out + 0.3 * (c + 1) * torch.log(Q[c])
@@ -80,14 +80,16 @@ def test_gradients_match_central_fd(backend, name, step):
 
     cp0 = torch.tensor(np.array(s.cell_param, dtype=np.float64))
     L0 = adjoint.block_lengths(s)
+    p0 = torch.tensor(np.array(s.phi_chain, dtype=np.float64))
     om = omg.clone().requires_grad_(True)
     cp = cp0.clone().requires_grad_(True)
     Lb = L0.clone().requires_grad_(True)
-    loss(om, cp, Lb).backward()
+    ph = p0.clone().requires_grad_(True)
+    loss(om, cp, Lb, ph).backward()
 
     d = torch.tensor(rng.normal(size=omg.shape), device=dev)   # arbitrary, not symmetric
     with torch.no_grad():
-        fd = _fd(lambda h: loss(omg + h * d, cp0, L0).item(), H)
+        fd = _fd(lambda h: loss(omg + h * d, cp0, L0, p0).item(), H)
     ad = (om.grad * d).sum().item()
     assert abs(ad - fd) <= FD_TOL * abs(fd), ("omega", ad, fd)
 
@@ -96,7 +98,7 @@ def test_gradients_match_central_fd(backend, name, step):
         e[p] = 1.0
         h = H * cp0[p].item()
         with torch.no_grad():
-            fd = _fd(lambda t: loss(omg, cp0 + t * e, L0).item(), h)
+            fd = _fd(lambda t: loss(omg, cp0 + t * e, L0, p0).item(), h)
         assert abs(cp.grad[p].item() - fd) <= FD_TOL * abs(fd), ("cell", p, cp.grad[p].item(), fd)
 
     for b in range(len(L0)):
@@ -104,9 +106,17 @@ def test_gradients_match_central_fd(backend, name, step):
         e[b] = 1.0
         h = H * L0[b].item()
         with torch.no_grad():
-            fd = _fd(lambda t: loss(omg, cp0, L0 + t * e).item(), h)
+            fd = _fd(lambda t: loss(omg, cp0, L0 + t * e, p0).item(), h)
         assert abs(Lb.grad[b].item() - fd) <= FD_TOL * max(abs(fd), 1e-3), ("length", b,
                                                                            Lb.grad[b].item(), fd)
+
+    for c in range(len(p0)):        # chain volume fractions, each varied alone
+        e = torch.zeros_like(p0)
+        e[c] = 1.0
+        h = H * p0[c].item()
+        with torch.no_grad():
+            fd = _fd(lambda t: loss(omg, cp0, L0, p0 + t * e).item(), h)
+        assert abs(ph.grad[c].item() - fd) <= FD_TOL * abs(fd), ("phi", c, ph.grad[c].item(), fd)
 
 
 def test_free_energy_gradient_matches_fd(backend):
@@ -252,6 +262,49 @@ def test_one_workgroup_step_kernel_adjoint_emu(monkeypatch):
     out = _loss_and_grads(S1, s1, omg1, dev1)
     for nm, a, b in zip(["rho", "Q", "grad_omega", "grad_cell", "grad_length"], out, ref):
         assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max(), nm
+
+
+@pytest.mark.gpu
+def test_blend96_gradients_checkpointed_gpu():
+    """ABA triblock (N=100, blocks 20/60/20) blended with A homopolymer (length
+    50, phi 0.12), chi N = 20, ds = 0.5, on a 96^3 grid, q checkpointed every 14
+    steps: gradients of sum_r (phi_A - t)^2 in f_A (the triblock's A fraction,
+    N fixed) and in the homopolymer fraction (phi = 1 - phi_h, phi_h) against
+    central FD, from a seeded random omega grid."""
+    import ctypes  # noqa: F401
+    lib = get_lib("gpu")
+    n = 96
+    chi = 0.2
+    s = System(kuhn=[1.0, 1.0], chains=[[(1, 20.0), (2, 60.0), (1, 20.0)], [(1, 50.0)]],
+               phi_chain=[0.88, 0.12], chi=np.array([[0.0, chi], [chi, 0.0]]), dim=3,
+               crystal_system="cubic", cell_param=[24.0], ngrid=[n, n, n], chain_step=0.5,
+               group_name="P 1")
+    G = np.eye(3) * (2.0 * np.pi / 24.0)
+    from pscf_amd import scf
+    basis = scf.Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
+    S = scf.SCFSystem(s, basis, G, lib=lib)
+    S.set_checkpoint(14)
+    rng = np.random.default_rng(96)
+    omg = torch.tensor(0.1 * rng.standard_normal((2, n, n, n)), device="cuda")
+    tgt = torch.tensor(0.5 + 0.1 * rng.standard_normal((n, n, n)), device="cuda")
+    cp0 = torch.tensor([24.0], dtype=torch.float64)
+
+    def loss(fA, phh):
+        L = torch.stack([50.0 * fA, 100.0 * (1.0 - fA), 50.0 * fA, torch.tensor(50.0,
+                         dtype=torch.float64)])
+        rho, Q = adjoint.density(S, omg, cp0, L, torch.stack([1.0 - phh, phh]))
+        return ((rho[0] - tgt) ** 2).sum()
+
+    f0 = torch.tensor(0.4, dtype=torch.float64)
+    h0 = torch.tensor(0.12, dtype=torch.float64)
+    fA = f0.clone().requires_grad_(True)
+    phh = h0.clone().requires_grad_(True)
+    loss(fA, phh).backward()
+    with torch.no_grad():
+        fd_f = _fd(lambda t: loss(f0 + t, h0).item(), H)
+        fd_h = _fd(lambda t: loss(f0, h0 + t).item(), H * 0.12)
+    assert abs(fA.grad.item() - fd_f) <= FD_TOL * abs(fd_f), (fA.grad.item(), fd_f)
+    assert abs(phh.grad.item() - fd_h) <= FD_TOL * abs(fd_h), (phh.grad.item(), fd_h)
 
 
 @pytest.mark.gpu
