@@ -238,6 +238,50 @@ def adjoint_leg(L, g, omega, cell_a, t_solve, reps=3):
             "loss": "sum_r (phi_A - phi_t)^2, gyroid 64^3 at the converged state"}
 
 
+def jacobian_leg(L, cuts=(17, 33), reps=3):
+    """Newton-Raphson Jacobian of the BCC 32^3 melt (N=100, f=0.25, chi N=20,
+    a=18): the N_cut * N_monomer perturbed density() calls of
+    response_pd_omega (src/response_pd_mod.f90:114-124, increment 1e-8), one
+    by one as the reference makes them and grouped through density_many
+    (one launch set), each at the seeded star field; the results must agree
+    bit for bit."""
+    from pscf_amd import scf
+    from pscf_amd.system import diblock
+    from pscf_amd import iterate
+    d = np.load(os.path.join(ROOT, "tests", "golden", "bcc32.npz"))
+    g = {k: d[k] for k in d.files}
+    sysm = diblock(0.25, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[18.0],
+                   ngrid=[32, 32, 32], chain_step=1.0, group_name="I m -3 m")
+    _, G, _ = iterate.unit_cell(L, 3, "cubic", [18.0])
+    S = scf.SCFSystem(sysm, scf.Basis.from_tables(g), G, lib=L)
+    om = np.asarray(g["omega"], dtype=np.float64)
+    out = []
+    for cut in cuts:
+        fields = []
+        for j in range(1, cut):
+            for beta in range(2):
+                w = om.copy()
+                w[beta, j] += 1.0e-8
+                fields.append(w)
+        fields = np.array(fields)
+        t_one, t_many = [], []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            one = [S.density(w, with_q=False)[0] for w in fields]
+            t1 = time.perf_counter()
+            many = S.density_many(fields, with_q=False)[0]
+            t2 = time.perf_counter()
+            t_one.append(t1 - t0)
+            t_many.append(t2 - t1)
+        same = all(np.array_equal(a, b) for a, b in zip(one, many))
+        a_ms = float(np.median(t_one[1:])) * 1e3
+        b_ms = float(np.median(t_many[1:])) * 1e3
+        out.append({"n_cut": cut, "density_calls": len(fields), "serial_ms": a_ms,
+                    "grouped_ms": b_ms, "speedup": a_ms / b_ms, "bitwise_equal": bool(same)})
+    return {"metric": "Newton-Raphson Jacobian (perturbed density() calls)",
+            "workload": "BCC 32^3 AB diblock, N=100, f=0.25, chi N=20, ds=1", "cases": out}
+
+
 def traffic_probe():
     import torch
     from pscf_amd import iterate, scf
@@ -304,6 +348,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient-cost leg")
+    ap.add_argument("--no-jacobian", action="store_true", help="skip the Newton Jacobian leg")
     ap.add_argument("--task-scaling", action="store_true",
                     help="only print contour-step cost versus sweeps per launch (BCC 64^3)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -374,6 +419,8 @@ def main():
     extra = {}
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_leg(L, g, r.omega, r.cell_param[0], elapsed / args.steps)
+    if not args.no_jacobian:
+        extra["jacobian"] = jacobian_leg(L)
     if not args.no_batch:
         extra["batch"] = batch_leg(L)
 

@@ -114,9 +114,24 @@ int pscf_system_startup(int64_t sys, double chain_step);
  * omega, rho: (N_monomer, N_star).  phi_chain/mu_chain(N_chain) and
  * phi_solvent/mu_solvent(N_solvent) are the chemistry_mod globals: read
  * (canonical) or updated via mu_phi_* (grand ensemble, :202-209).
- * qout(N_chain), q_solvent(N_solvent) may be NULL. */
+ * qout(N_chain), q_solvent(N_solvent) may be NULL, as the optional
+ * arguments of the reference: without qout the grand ensemble does not
+ * update phi_chain, and the chain densities take the given phi_chain
+ * (:202-218). */
 int pscf_density(int64_t sys, const double* omega, double* rho, double* qout, double* q_solvent,
                  double* phi_chain, double* mu_chain, double* phi_solvent, double* mu_solvent);
+
+/* n density() evaluations of one system at once: the perturbed fields of
+ * the finite-difference Jacobian (response_pd_omega,
+ * src/response_pd_mod.f90:91-147, N_cut * N_monomer density() calls).
+ * omega, rho: n consecutive (N_monomer, N_star) arrays; qout (N_chain, n)
+ * and q_solvent (N_solvent, n) may be NULL; phi/mu as pscf_density, in and
+ * out as n pscf_density calls in a row.  Each result equals the single
+ * call's.  max_batch > 0 caps the evaluations per launch set (0: up to 64,
+ * within 32 GiB of copies of the system's contour storage). */
+int pscf_density_many(int64_t sys, int n, const double* omega, double* rho, double* qout,
+                      double* q_solvent, double* phi_chain, double* mu_chain, double* phi_solvent,
+                      double* mu_solvent, int max_batch);
 
 /* scf_stress(N, size_dGsq, dGsq)                 src/scf_mod.fp.f90:451-550
  * dGsq(N_star, ncp); uses the slabs of the last density().  stress(ncp). */

@@ -50,6 +50,8 @@ SIGNATURES = {
     "pscf_set_omega_uniform": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pscf_density": [ctypes.c_int64, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                      _c_dbl_p, _c_dbl_p],
+    "pscf_density_many": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                          _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int],
     "pscf_scf_stress": [ctypes.c_int64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pscf_density_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _c_dbl_p, _c_dbl_p,
                               _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],

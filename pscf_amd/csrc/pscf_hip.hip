@@ -354,6 +354,8 @@ struct AdjState {
   }
 };
 
+struct Multi;
+
 struct System {
   Grid g;
   Stream st;
@@ -362,7 +364,7 @@ struct System {
   DevBuf<int> star_of_wave, star_invert, wtab, kwave, sconj;
   DevBuf<double2> coeff, scoef;
   DevBuf<long> sidx;
-  std::vector<int> h_wave, h_wave_of_star;
+  std::vector<int> h_wave, h_wave_of_star, h_star_of_wave, h_star_invert;
   std::vector<double2> h_coeff;
   int gmin[3] = {0, 0, 0}, gext[3] = {1, 1, 1};
   bool have_basis = false, have_cell = false;
@@ -385,6 +387,15 @@ struct System {
   std::vector<StepTask> h_tasks;
   std::vector<double> h_Q, h_Qs, h_chem, h_dots;
   bool have_density = false;
+  // ensemble of the chain prefactors of the last density(): the reference
+  // updates phi_chain = Q e^mu first only when qout is present
+  // (src/scf_mod.fp.f90:202-205); otherwise rho uses the given phi_chain
+  int chain_ens = 0;
+  // generations of the basis and of the chain discretisation (density_many's
+  // copies of the system are rebuilt when they change)
+  uint64_t gen_basis = 0, gen_start = 0;
+  std::unique_ptr<Multi> multi;
+  ~System();
   // contour sweeps of one density() captured once as a HIP graph
   int ntask = 0, max_ns = 0;
   hipGraph_t graph = nullptr;
@@ -811,6 +822,7 @@ void startup(System& S, double chain_step) {
   S.adj.reset();
   S.started = true;
   S.have_density = false;
+  ++S.gen_start;
   if (S.ck_stride > 0) build_ck_plan(S);
 }
 
@@ -818,11 +830,15 @@ void startup(System& S, double chain_step) {
 
 // omega stars -> grid (no 1/M, :182-185), propagator factors per block
 // (make_propg), chemistry scalars for the device
-// (grid_in: S.omega_grid already holds the omega grids, star input unused)
+// (grid_in: S.omega_grid already holds the omega grids, star input unused;
+// chain_update false: density() without qout, whose chain densities take the
+// given phi_chain in either ensemble)
 void density_pre(System& S, const double* omega, const double* phi_c, const double* mu_c,
-                 const double* phi_s, const double* mu_s, hipStream_t s, bool grid_in = false) {
+                 const double* phi_s, const double* mu_s, hipStream_t s, bool grid_in = false,
+                 bool chain_update = true) {
   REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
   const Grid& g = S.g;
+  S.chain_ens = (S.ensemble == 1 && chain_update) ? 1 : 0;
   if (!grid_in) {
     const size_t nst = (size_t)S.nmon * S.nstar;
     S.omega_in.upload(omega, nst, s);
@@ -921,7 +937,7 @@ void density_post(System& S, double* rho, hipStream_t s) {
       a.monomer[b] = ch.mon[b];
     }
     a.chain_length = ch.length;
-    a.ensemble = S.ensemble;
+    a.ensemble = S.chain_ens;
     if (S.ck_stride > 0)
       hipLaunchKernelGGL(k_block_rho, dim3(grid_blocks(g.m)), dim3(NTA), 0, s, ch.blk.p, g.m, a,
                          S.qdev.p + c, S.chem_dev.p + 2 * c, S.rho_grid.p);
@@ -951,7 +967,7 @@ void density_finish(System& S, double* qout, double* qsolv, double* phi_c, doubl
     if (qsolv) qsolv[i] = S.h_Q[S.nchain + i];
   if (S.ensemble == 1) {
     for (int c = 0; c < S.nchain; ++c)
-      if (phi_c && mu_c) phi_c[c] = S.h_Q[c] * std::exp(mu_c[c]);
+      if (phi_c && mu_c && S.chain_ens == 1) phi_c[c] = S.h_Q[c] * std::exp(mu_c[c]);
     for (int i = 0; i < S.nsolv; ++i)
       if (phi_s && mu_s) phi_s[i] = S.h_Q[S.nchain + i] * std::exp(mu_s[i]);
   }
@@ -961,7 +977,7 @@ void density_finish(System& S, double* qout, double* qsolv, double* phi_c, doubl
 void density(System& S, const double* omega, double* rho, double* qout, double* qsolv, double* phi_c,
              double* mu_c, double* phi_s, double* mu_s) {
   hipStream_t s = S.st.s;
-  density_pre(S, omega, phi_c, mu_c, phi_s, mu_s, s);
+  density_pre(S, omega, phi_c, mu_c, phi_s, mu_s, s, false, qout != nullptr);
   density_sweeps(S, s);
   density_post(S, rho, s);
   S.st.sync();
@@ -976,12 +992,174 @@ void density_grid(System& S, const double* omega_grid, double* rho_grid, double*
   hipStream_t s = S.st.s;
   const size_t n = (size_t)S.nmon * S.g.m;
   HIPCHK(hipMemcpyAsync(S.omega_grid.p, omega_grid, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  density_pre(S, nullptr, phi_c, mu_c, phi_s, mu_s, s, true);
+  density_pre(S, nullptr, phi_c, mu_c, phi_s, mu_s, s, true, qout != nullptr);
   density_sweeps(S, s);
   density_post(S, nullptr, s);
   HIPCHK(hipMemcpyAsync(rho_grid, S.rho_grid.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   S.st.sync();
   density_finish(S, qout, qsolv, phi_c, mu_c, phi_s, mu_s);
+}
+
+// ---- many density() evaluations of one system in one launch set
+//
+// The finite-difference Jacobian of Newton-Raphson iteration
+// (src/response_pd_mod.f90:91-147) evaluates density() at N_cut * N_monomer
+// perturbed omega fields of one system.  density_many runs such a list
+// through copies of the system (same grid, stars, cell, chemistry and
+// contour discretisation) whose sweep tasks share one launch per contour step,
+// captured once as a HIP graph; every copy runs the kernels of a lone
+// density(), so each result equals the corresponding single call.
+
+struct Multi {
+  std::vector<std::unique_ptr<System>> cl;
+  uint64_t gen_basis = 0, gen_start = 0;
+  std::vector<StepTask> h_tasks;
+  std::vector<int> owner;        // copy of every task
+  DevBuf<StepTask> tasks;
+  int max_ns = 0, nlive = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~Multi() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+System::~System() = default;
+
+// device bytes of one copy of S (contour storage, factors, step work)
+size_t system_bytes(const System& S) {
+  const Grid& g = S.g;
+  size_t b = 0;
+  for (const auto& ch : S.chains) {
+    b += (size_t)2 * ch->nslot * g.m * 8 + (size_t)ch->nblk * (2 * g.m + 2 * g.mk) * 8;
+    if (S.ck_stride > 0) b += (size_t)(4 * ch->segpts + ch->nblk) * g.m * 8;
+  }
+  b += (size_t)2 * S.nchain * (3 * g.mk * 16 + g.m * 8);
+  b += (size_t)(2 * S.nmon + 4) * g.m * 8 + (size_t)2 * S.nmon * g.mk * 16;
+  return b;
+}
+
+// a copy of S's grid, stars, cell, chemistry and discretisation
+void copy_system(System& D, const System& S) {
+  const int ng[3] = {S.g.n1, S.g.n2, S.g.n3};
+  D.g.init(ng, S.g.dim);
+  D.nwave = S.nwave;
+  D.nstar = S.nstar;
+  D.h_wave = S.h_wave;
+  D.h_wave_of_star = S.h_wave_of_star;
+  D.h_star_of_wave = S.h_star_of_wave;
+  D.h_star_invert = S.h_star_invert;
+  D.h_coeff = S.h_coeff;
+  D.coeff.upload(D.h_coeff.data(), D.nwave, D.st.s);
+  D.star_of_wave.upload(D.h_star_of_wave.data(), D.nwave, D.st.s);
+  D.star_invert.upload(D.h_star_invert.data(), D.nstar, D.st.s);
+  build_basis_maps(D);
+  D.have_basis = true;
+  for (int i = 0; i < 9; ++i) D.gb[i] = S.gb[i];
+  apply_cell(D);
+  D.nmon = S.nmon;
+  D.nchain = S.nchain;
+  D.nsolv = S.nsolv;
+  D.ensemble = S.ensemble;
+  D.kuhn = S.kuhn;
+  D.blocks = S.blocks;
+  D.solv_mon = S.solv_mon;
+  D.solv_size = S.solv_size;
+  D.chi = S.chi;
+  D.have_chem = true;
+  D.ck_stride = S.ck_stride;
+  D.use_graph = S.use_graph;
+  startup(D, S.chain_step);
+}
+
+// at least P copies matching S's current state (cell refreshed on every call)
+void ensure_multi(System& S, int P) {
+  if (S.multi && (S.multi->gen_basis != S.gen_basis || S.multi->gen_start != S.gen_start ||
+                  (int)S.multi->cl.size() < P))
+    S.multi.reset();
+  if (!S.multi) {
+    auto M = std::make_unique<Multi>();
+    M->gen_basis = S.gen_basis;
+    M->gen_start = S.gen_start;
+    for (int i = 0; i < P; ++i) {
+      M->cl.push_back(std::make_unique<System>());
+      copy_system(*M->cl.back(), S);
+      const System& D = *M->cl.back();
+      for (const auto& t : D.h_tasks) {
+        M->h_tasks.push_back(t);
+        M->owner.push_back(i);
+      }
+      M->max_ns = std::max(M->max_ns, D.max_ns);
+    }
+    M->tasks.alloc(std::max<size_t>(M->h_tasks.size(), 1));
+    for (auto& D : M->cl) D->st.sync();
+    S.multi = std::move(M);
+  }
+  for (auto& D : S.multi->cl) {
+    bool same = true;
+    for (int i = 0; i < 9; ++i) same = same && D->gb[i] == S.gb[i];
+    if (!same) {
+      for (int i = 0; i < 9; ++i) D->gb[i] = S.gb[i];
+      apply_cell(*D);
+      D->st.sync();
+    }
+    D->use_graph = S.use_graph;
+  }
+}
+
+// omega (n consecutive (N_monomer, N_star) arrays) -> rho (same), qout
+// (N_chain per evaluation) and q_solvent (N_solvent per evaluation), each as
+// density(); phi/mu in and out as n density() calls in a row.  max_batch > 0
+// caps the evaluations per launch set (default: 64 copies, within 32 GiB).
+void density_many(System& S, int n, const double* omega, double* rho, double* qout, double* qsolv,
+                  double* phi_c, double* mu_c, double* phi_s, double* mu_s, int max_batch) {
+  REQUIRE(S.started && S.have_basis && S.have_cell, "system not ready (basis, cell, startup)");
+  REQUIRE(n >= 0 && max_batch >= 0, "negative count");
+  if (n == 0) return;
+  REQUIRE(omega && rho, "null omega or rho");
+  int cap = max_batch;
+  if (cap == 0) {
+    const size_t per = std::max<size_t>(system_bytes(S), 1);
+    cap = (int)std::min<size_t>(64, std::max<size_t>(1, ((size_t)32 << 30) / per));
+  }
+  const int P = std::min(n, cap);
+  ensure_multi(S, P);
+  Multi& Mu = *S.multi;
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  const size_t nst = (size_t)S.nmon * S.nstar;
+  const bool upd = qout != nullptr;
+  for (int k0 = 0; k0 < n; k0 += P) {
+    const int cnt = std::min(P, n - k0);
+    if (cnt != Mu.nlive) {         // copies past cnt sit out this launch set
+      for (size_t t = 0; t < Mu.h_tasks.size(); ++t) Mu.h_tasks[t].skip = Mu.owner[t] < cnt ? 0 : 1;
+      HIPCHK(hipMemcpyAsync(Mu.tasks.p, Mu.h_tasks.data(), sizeof(StepTask) * Mu.h_tasks.size(),
+                            hipMemcpyHostToDevice, s));
+      Mu.nlive = cnt;
+    }
+    for (int i = 0; i < cnt; ++i)
+      density_pre(*Mu.cl[i], omega + (size_t)(k0 + i) * nst, phi_c, mu_c, phi_s, mu_s, s, false, upd);
+    const int nt = (int)Mu.h_tasks.size();
+    if (nt > 0) {
+      if (S.use_graph) {
+        if (!Mu.exec) {
+          HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          run_sweeps(g, Mu.tasks.p, nt, Mu.max_ns, s);
+          HIPCHK(hipStreamEndCapture(s, &Mu.graph));
+          HIPCHK(hipGraphInstantiate(&Mu.exec, Mu.graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(Mu.exec, s));
+      } else {
+        run_sweeps(g, Mu.tasks.p, nt, Mu.max_ns, s);
+      }
+    }
+    for (int i = 0; i < cnt; ++i) density_post(*Mu.cl[i], rho + (size_t)(k0 + i) * nst, s);
+    S.st.sync();
+    for (int i = 0; i < cnt; ++i)
+      density_finish(*Mu.cl[i], qout ? qout + (size_t)(k0 + i) * S.nchain : nullptr,
+                     qsolv ? qsolv + (size_t)(k0 + i) * S.nsolv : nullptr, phi_c, mu_c, phi_s, mu_s);
+  }
 }
 
 // ---- reverse mode of density()
@@ -1225,7 +1403,7 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
       a.monomer[b] = ch.mon[b];
     }
     a.chain_length = ch.length;
-    a.ensemble = S.ensemble;
+    a.ensemble = S.chain_ens;
     T[c].resize(ch.nblk);
     if (S.ck_stride > 0) {      // the block integrals of the last density() are kept
       for (int b = 0; b < ch.nblk; ++b) {
@@ -1239,11 +1417,11 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
       for (int b = 0; b < ch.nblk; ++b) T[c][b] = host_sum(S, part + (size_t)b * nb, nb);
     }
     const double Q = S.h_Q[c];
-    coef[c] = S.ensemble == 1 ? std::exp(S.h_chem[2 * c + 1]) / ch.length
-                              : S.h_chem[2 * c] / (ch.length * Q);
+    coef[c] = S.chain_ens == 1 ? std::exp(S.h_chem[2 * c + 1]) / ch.length
+                               : S.h_chem[2 * c] / (ch.length * Q);
     double tsum = 0.0;
     for (int b = 0; b < ch.nblk; ++b) tsum += T[c][b];
-    gam[c] = (dQ ? dQ[c] : 0.0) - (S.ensemble == 1 ? 0.0 : coef[c] * tsum / Q);
+    gam[c] = (dQ ? dQ[c] : 0.0) - (S.chain_ens == 1 ? 0.0 : coef[c] * tsum / Q);
   }
   // sources: lam copied into the adjoint state, coefficient and end term per chain
   HIPCHK(hipMemcpyAsync(A.lam.p, lam, sizeof(double) * S.nmon * g.m, hipMemcpyDeviceToDevice, s));
@@ -1307,7 +1485,7 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
       double tsum = 0.0;
       for (double t : T[c]) tsum += t;
       const double phi = S.h_chem[2 * c];
-      dphi[c] = (S.ensemble == 0 && phi != 0.0) ? coef[c] * tsum / phi : 0.0;
+      dphi[c] = (S.chain_ens == 0 && phi != 0.0) ? coef[c] * tsum / phi : 0.0;
     }
   }
   if (dlen) {
@@ -1952,11 +2130,14 @@ int pscf_system_set_basis(int64_t sys, int n_wave, int n_star, const int* wave, 
     }
     for (int i = 0; i < n_wave; ++i)
       REQUIRE(star_of_wave[i] >= 1 && star_of_wave[i] <= n_star, "star_of_wave out of range");
+    S.h_star_of_wave.assign(star_of_wave, star_of_wave + n_wave);
+    S.h_star_invert.assign(star_invert, star_invert + n_star);
     S.coeff.upload(S.h_coeff.data(), n_wave, S.st.s);
     S.star_of_wave.upload(star_of_wave, n_wave, S.st.s);
     S.star_invert.upload(star_invert, n_star, S.st.s);
     build_basis_maps(S);
     S.have_basis = true;
+    ++S.gen_basis;
     if (S.have_cell) {
       apply_cell(S);
       S.st.sync();
@@ -2139,6 +2320,15 @@ int pscf_density(int64_t sys, const double* omega, double* rho, double* qout, do
   return guarded([&] {
     density(*from_handle<System>(sys, "system"), omega, rho, qout, q_solvent, phi_chain, mu_chain,
             phi_solvent, mu_solvent);
+  });
+}
+
+int pscf_density_many(int64_t sys, int n, const double* omega, double* rho, double* qout,
+                      double* q_solvent, double* phi_chain, double* mu_chain, double* phi_solvent,
+                      double* mu_solvent, int max_batch) {
+  return guarded([&] {
+    density_many(*from_handle<System>(sys, "system"), n, omega, rho, qout, q_solvent, phi_chain,
+                 mu_chain, phi_solvent, mu_solvent, max_batch);
   });
 }
 

@@ -13,7 +13,7 @@ module pscf_hip_c
    public :: c_pscf_chain_discretize
    public :: c_pscf_system_create, c_pscf_system_destroy, c_pscf_system_set_basis
    public :: c_pscf_system_set_cell, c_pscf_system_set_chemistry, c_pscf_system_set_chi
-   public :: c_pscf_system_startup, c_pscf_density, c_pscf_scf_stress
+   public :: c_pscf_system_startup, c_pscf_density, c_pscf_density_many, c_pscf_scf_stress
    public :: c_pscf_mu_phi_chain, c_pscf_mu_phi_solvent, c_pscf_free_energy
    public :: c_pscf_free_energy_fh, c_pscf_set_omega_uniform, c_pscf_divide_energy_terms
    public :: c_pscf_system_get_grids, c_pscf_basis_to_kgrid, c_pscf_kgrid_to_basis
@@ -124,13 +124,27 @@ module pscf_hip_c
          integer(c_int64_t), value :: sys
          real(c_double), value :: chain_step
       end function
+      ! qout, q_solvent: c_loc of an array, or c_null_ptr for an absent optional
       integer(c_int) function c_pscf_density(sys, omega, rho, qout, q_solvent, phi_chain, &
             mu_chain, phi_solvent, mu_solvent) bind(C, name='pscf_density')
-         import :: c_int, c_int64_t, c_double
+         import :: c_int, c_int64_t, c_double, c_ptr
          integer(c_int64_t), value :: sys
          real(c_double), intent(IN) :: omega(*)
          real(c_double), intent(OUT) :: rho(*)
-         real(c_double), intent(INOUT) :: qout(*), q_solvent(*), phi_chain(*), mu_chain(*)
+         type(c_ptr), value :: qout, q_solvent
+         real(c_double), intent(INOUT) :: phi_chain(*), mu_chain(*)
+         real(c_double), intent(INOUT) :: phi_solvent(*), mu_solvent(*)
+      end function
+      integer(c_int) function c_pscf_density_many(sys, n, omega, rho, qout, q_solvent, &
+            phi_chain, mu_chain, phi_solvent, mu_solvent, max_batch) &
+            bind(C, name='pscf_density_many')
+         import :: c_int, c_int64_t, c_double, c_ptr
+         integer(c_int64_t), value :: sys
+         integer(c_int), value :: n, max_batch
+         real(c_double), intent(IN) :: omega(*)
+         real(c_double), intent(OUT) :: rho(*)
+         type(c_ptr), value :: qout, q_solvent
+         real(c_double), intent(INOUT) :: phi_chain(*), mu_chain(*)
          real(c_double), intent(INOUT) :: phi_solvent(*), mu_solvent(*)
       end function
       integer(c_int) function c_pscf_scf_stress(sys, ncp, dGsq, phi_chain, mu_chain, stress) &

@@ -1,0 +1,288 @@
+!-----------------------------------------------------------------------
+! response_pd_mod -- drop-in replacement of src/response_pd_mod.f90 with the
+! same public procedures, arguments and module array `corrlt`.
+!
+! The Newton-Raphson Jacobian of iterate_NR (src/iterate_mod.fp.f90:878)
+! differentiates density() by finite differences: one perturbed omega per
+! star j <= cut and monomer beta (src/response_pd_mod.f90:114-124).  Those
+! N_cut * N_monomer evaluations are independent, so response_pd_omega here
+! hands them to scf_mod's density_many in groups, one launch set of the
+! contour-step kernels per group, instead of one density() call each.  The
+! perturbed fields, the difference quotients and the fill of the rest of
+! the matrix are those of the reference, and density_many returns exactly
+! what the single calls return, so the Jacobian is the same matrix.
+!
+! The other procedures keep the reference's arithmetic: the Debye
+! correlation functions of ideal chains (make_correlation, used by the
+! short-wavelength block of the Jacobian and by spinodal_mod) and the
+! cell-parameter responses by finite differences (response_pd_cell,
+! response_dstress_dcell; N_cell_param <= 6 serial density() calls).
+!-----------------------------------------------------------------------
+module response_pd_mod
+   use const_mod
+   implicit none
+
+   private
+   public :: init_response_pd
+   public :: response_pd_omega
+   public :: response_pd_cell
+   public :: make_correlation
+   public :: corrlt
+   public :: response_dstress_dcell
+
+   ! corrlt(alpha, beta, i_star): ideal-chain correlation of monomers alpha
+   ! and beta at the wavenumber of star i_star
+   real(long), allocatable :: corrlt(:,:,:)
+
+   ! finite-difference increment of omega coefficients and cell parameters
+   real(long), parameter :: increment = 1.0d-8
+   real(long), parameter :: inv_increment = 1.0_long / increment
+   ! perturbed density() evaluations per density_many call
+   integer, parameter    :: max_group = 64
+
+contains
+
+   !--------------------------------------------------------------------
+   ! init_response_pd(N_monomer, N)          src/response_pd_mod.f90:55-66
+   !--------------------------------------------------------------------
+   subroutine init_response_pd(N_monomer, N)
+   integer :: N_monomer, N
+   integer :: info
+   if (.not. allocated(corrlt)) then
+      allocate(corrlt(N_monomer, N_monomer, N), stat=info)
+      if (info /= 0) stop "Error allocating corrlt"
+   end if
+   end subroutine init_response_pd
+
+   !--------------------------------------------------------------------
+   ! response_pd_omega(N, cut, omega, drho_domega)
+   !                                        src/response_pd_mod.f90:91-147
+   ! drho_domega(alpha,i,beta,j) = d rho(alpha,i) / d omega(beta,j): stars
+   ! i, j <= cut by forward differences of density(), the columns j > cut
+   ! of those rows by symmetry, and the block i, j > cut from corrlt.
+   !--------------------------------------------------------------------
+   subroutine response_pd_omega(N, cut, omega, drho_domega)
+   use chemistry_mod, only : ensemble, N_monomer
+   use scf_mod,       only : density, density_many
+   integer, intent(IN)     :: N
+   integer, intent(IN)     :: cut
+   real(long), intent(IN)  :: omega(:,:)
+   real(long), intent(OUT) :: drho_domega(:,:,:,:)
+
+   real(long)              :: rho(N_monomer, N)
+   real(long), allocatable :: w(:,:,:), p(:,:,:)
+   integer :: first, n_eval, n_group, k0, k, cnt, j, beta, i, alpha, next
+
+   first = 2 - ensemble
+   call density(N, omega, rho)
+   drho_domega = 0.0_long
+
+   if (cut > 1) then
+      ! evaluation k (0-based) perturbs omega(beta, j), j = first + k / N_monomer,
+      ! beta = 1 + mod(k, N_monomer): the reference's loop order
+      n_eval = (cut - first + 1) * N_monomer
+      n_group = min(n_eval, max_group)
+      allocate(w(N_monomer, N, n_group), p(N_monomer, N, n_group))
+      do k0 = 0, n_eval - 1, n_group
+         cnt = min(n_group, n_eval - k0)
+         do k = 1, cnt
+            j = first + (k0 + k - 1) / N_monomer
+            beta = 1 + mod(k0 + k - 1, N_monomer)
+            w(:, :, k) = omega(:, 1:N)
+            w(beta, j, k) = w(beta, j, k) + increment
+         end do
+         call density_many(N, cnt, w, p)
+         do k = 1, cnt
+            j = first + (k0 + k - 1) / N_monomer
+            beta = 1 + mod(k0 + k - 1, N_monomer)
+            drho_domega(:, first:, beta, j) = inv_increment * (p(:, first:N, k) - rho(:, first:N))
+         end do
+      end do
+      deallocate(w, p)
+      ! columns past the cut of the computed rows, by the symmetry of the response
+      do j = cut + 1, N
+         do beta = 1, N_monomer
+            do i = first, cut
+               do alpha = 1, N_monomer
+                  drho_domega(alpha, i, beta, j) = drho_domega(beta, j, alpha, i)
+               end do
+            end do
+         end do
+      end do
+      next = cut + 1
+   else
+      next = first
+   end if
+
+   ! short wavelengths: the ideal-chain (Debye) response, diagonal in the star
+   do j = next, N
+      do beta = 1, N_monomer
+         drho_domega(:, j, beta, j) = -corrlt(:, beta, j)
+      end do
+   end do
+   end subroutine response_pd_omega
+
+   !--------------------------------------------------------------------
+   ! response_pd_cell(N, omega, drho_dcell, dstress_dcell)
+   !                                        src/response_pd_mod.f90:276-339
+   !--------------------------------------------------------------------
+   subroutine response_pd_cell(N, omega, drho_dcell, dstress_dcell)
+   use chemistry_mod, only : ensemble, N_monomer
+   use unit_cell_mod, only : N_cell_param
+   integer, intent(IN)     :: N
+   real(long), intent(IN)  :: omega(:,:)
+   real(long), intent(OUT) :: drho_dcell(:,:,:)
+   real(long), intent(OUT) :: dstress_dcell(:,:)
+   call cell_response(N, omega, dstress_dcell, drho_dcell, 2 - ensemble)
+   end subroutine response_pd_cell
+
+   !--------------------------------------------------------------------
+   ! response_dstress_dcell(N, omega, dstress_dcell)
+   !                                        src/response_pd_mod.f90:351-407
+   !--------------------------------------------------------------------
+   subroutine response_dstress_dcell(N, omega, dstress_dcell)
+   integer, intent(IN)     :: N
+   real(long), intent(IN)  :: omega(:,:)
+   real(long), intent(OUT) :: dstress_dcell(:,:)
+   call cell_response(N, omega, dstress_dcell)
+   end subroutine response_dstress_dcell
+
+   ! forward differences in each cell parameter of the stress and (with
+   ! drho_dcell) of rho(:, first:), at fixed omega; the cell is restored after
+   subroutine cell_response(N, omega, dstress_dcell, drho_dcell, first)
+   use chemistry_mod, only : N_monomer
+   use scf_mod,       only : density, scf_stress
+   use grid_mod,      only : make_ksq
+   use basis_mod,     only : make_dGsq
+   use unit_cell_mod, only : N_cell_param, cell_param, G_basis, dGG_basis, make_unit_cell
+   integer, intent(IN)              :: N
+   real(long), intent(IN)           :: omega(:,:)
+   real(long), intent(OUT)          :: dstress_dcell(:,:)
+   real(long), intent(OUT), optional :: drho_dcell(:,:,:)
+   integer, intent(IN), optional    :: first
+
+   real(long), dimension(N_monomer, N)    :: rho, p
+   real(long), dimension(N_cell_param)    :: base, stress, stress1
+   real(long), dimension(N, N_cell_param) :: dGsq
+   integer :: i
+
+   call make_unit_cell
+   call make_ksq(G_basis)
+   call density(N, omega, rho)
+   call all_dgsq(dGsq)
+   stress = scf_stress(N, N_cell_param, dGsq)
+
+   base = cell_param(1:N_cell_param)
+   do i = 1, N_cell_param
+      cell_param(1:N_cell_param) = base
+      cell_param(i) = base(i) + increment
+      call make_unit_cell
+      call make_ksq(G_basis)
+      call density(N, omega, p)
+      if (present(drho_dcell)) then
+         drho_dcell(:, first:, i) = inv_increment * (p(:, first:) - rho(:, first:))
+      end if
+      call all_dgsq(dGsq)
+      stress1 = scf_stress(N, N_cell_param, dGsq)
+      dstress_dcell(:, i) = inv_increment * (stress1 - stress)
+   end do
+
+   cell_param(1:N_cell_param) = base
+   call make_unit_cell
+   call make_ksq(G_basis)
+
+   contains
+
+      subroutine all_dgsq(d)
+      real(long), intent(OUT) :: d(:,:)
+      integer :: c
+      do c = 1, N_cell_param
+         call make_dGsq(d(:, c), dGG_basis(:, :, c))
+      end do
+      end subroutine all_dgsq
+
+   end subroutine cell_response
+
+   !--------------------------------------------------------------------
+   ! make_correlation(N, lambda)            src/response_pd_mod.f90:419-526
+   ! corrlt(:,:,i) for stars i = 1..N at q^2 = |G_i|^2 (or lambda(i)):
+   ! sum over chains of phi * N_chain-weighted Debye functions of the blocks,
+   !   self:  g1(x_a) f_a^2,
+   !   pairs: g2(x_a, x_b) f_a f_b prod_{a<c<b} exp(-x_c)   (both orders),
+   ! with x = L b^2 q^2 / 6, f = L / chain length.  phi_chain is scaled by the
+   ! chain length and back inside the star loop, as the reference does, so
+   ! its value after the call is the reference's to the last bit.
+   !--------------------------------------------------------------------
+   subroutine make_correlation(N, lambda)
+   use chemistry_mod, only : N_chain, N_block, block_monomer, block_length, chain_length, &
+                             kuhn, phi_chain
+   use group_mod,     only : operator(.dot.)
+   use basis_mod,     only : wave_of_star
+   use unit_cell_mod, only : G_basis
+   integer                        :: N
+   real(long), optional           :: lambda(:)
+
+   real(long), parameter :: eps = 1.0d-14
+   real(long) :: q6, x(size(block_length, 1)), f(size(block_length, 1)), gv(3)
+   real(long) :: pw, term
+   integer    :: s, c, a, b, ma, mb
+
+   corrlt = 0.0_long
+   do s = 1, N
+      if (present(lambda)) then
+         q6 = lambda(s)
+      else
+         gv = wave_of_star(:, s) .dot. G_basis
+         q6 = gv .dot. gv
+      end if
+      q6 = q6 / 6.0_long
+      do c = 1, N_chain
+         phi_chain(c) = phi_chain(c) * chain_length(c)
+         do a = 1, N_block(c)
+            x(a) = block_length(a, c) * kuhn(block_monomer(a, c))**2 * q6
+            f(a) = block_length(a, c) / chain_length(c)
+         end do
+         do a = 1, N_block(c)
+            ma = block_monomer(a, c)
+            corrlt(ma, ma, s) = corrlt(ma, ma, s) + g1(x(a)) * f(a)**2 * phi_chain(c)
+         end do
+         do a = 1, N_block(c)
+            ma = block_monomer(a, c)
+            pw = 1.0_long            ! propagator through the blocks between a and b
+            do b = a + 1, N_block(c)
+               if (b > a + 1) pw = pw * exp(-x(b - 1))
+               mb = block_monomer(b, c)
+               if (ma == mb) then
+                  corrlt(ma, ma, s) = corrlt(ma, ma, s) &
+                                    + 2.0_long * g2(x(a), x(b)) * pw * f(a) * f(b) * phi_chain(c)
+               else
+                  term = g2(x(a), x(b)) * pw * f(a) * f(b) * phi_chain(c)
+                  corrlt(ma, mb, s) = corrlt(ma, mb, s) + term
+                  corrlt(mb, ma, s) = corrlt(mb, ma, s) + term
+               end if
+            end do
+         end do
+         if (chain_length(c) > 1.0d-12) phi_chain(c) = phi_chain(c) / chain_length(c)
+      end do
+   end do
+
+   contains
+
+      ! Debye function of one block: 2 (e^-x + x - 1) / x^2
+      real(long) function g1(xx)
+      real(long) :: xx
+      g1 = 1.0_long
+      if (xx > eps) g1 = g1 * 2.0_long * (exp(-xx) + xx - 1.0_long) / xx / xx
+      end function g1
+
+      ! end-to-end factor of a block pair: (1 - e^-x1)/x1 (1 - e^-x2)/x2
+      real(long) function g2(x1, x2)
+      real(long) :: x1, x2
+      g2 = 1.0_long
+      if (x1 > eps) g2 = g2 * (1.0_long - exp(-x1)) / x1
+      if (x2 > eps) g2 = g2 * (1.0_long - exp(-x2)) / x2
+      end function g2
+
+   end subroutine make_correlation
+
+end module response_pd_mod

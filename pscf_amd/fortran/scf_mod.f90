@@ -10,6 +10,8 @@
 ! grid fields in grid_mod's omega_grid/rho_grid (src/scf_mod.fp.f90:182-218)
 ! when sync_grid_globals is .true. (the default), and updates phi_chain /
 ! phi_solvent through mu_phi_* in the grand ensemble (:202-209).
+! density_many (no reference counterpart) evaluates the perturbed fields of
+! the finite-difference Jacobian in one launch set.
 !-----------------------------------------------------------------------
 module scf_mod
    use, intrinsic :: iso_c_binding
@@ -28,6 +30,7 @@ module scf_mod
 
    public:: density_startup
    public:: density
+   public:: density_many
    public:: scf_stress
    public:: mu_phi_chain
    public:: mu_phi_solvent
@@ -73,11 +76,62 @@ contains
    real(long), intent(OUT)           :: rho(:,:)
    real(long), intent(OUT), optional :: qout(N_chain)
    real(long), intent(OUT), optional :: q_solvent(N_solvent)
-   real(c_double) :: q(max(N_chain,1)), qs(max(N_solvent,1))
+   real(c_double), target :: q(max(N_chain,1)), qs(max(N_solvent,1))
    real(c_double) :: phc(max(N_chain,1)), muc(max(N_chain,1))
    real(c_double) :: phs(max(N_solvent,1)), mus(max(N_solvent,1))
+   type(c_ptr)    :: qp
    call sync_cell()
    call sync_chi()
+   call chem_in(phc, muc, phs, mus)
+   ! without qout the chains keep the given phi_chain in the grand ensemble (:202-205)
+   qp = c_null_ptr
+   if (present(qout)) qp = c_loc(q)
+   call hip_check(c_pscf_density(hsys, omega, rho, qp, c_loc(qs), phc, muc, phs, mus), 'density')
+   if (present(qout)) qout(1:N_chain) = q(1:N_chain)
+   if (present(q_solvent)) q_solvent(1:N_solvent) = qs(1:N_solvent)
+   if (ensemble == 1) then
+      if (present(qout)) phi_chain(1:N_chain) = phc(1:N_chain)
+      ! solvent_density sets phi_solvent in the grand ensemble on every call (:285-289)
+      if (N_solvent > 0) phi_solvent(1:N_solvent) = phs(1:N_solvent)
+   end if
+   if (sync_grid_globals) then
+      call hip_check(c_pscf_system_get_grids(hsys, omega_grid, rho_grid), 'density')
+   end if
+   end subroutine density
+
+   !--------------------------------------------------------------------
+   ! density_many(N, n_eval, omega, rho): density(N, omega(:,:,k), rho(:,:,k))
+   ! without qout and q_solvent for k = 1..n_eval, as n_eval calls in a row,
+   ! through pscf_density_many (one launch set per group of evaluations).
+   ! No counterpart in the reference: it serves the perturbed density() calls
+   ! of the finite-difference Jacobian (src/response_pd_mod.f90:114-124).
+   ! omega, rho: (N_monomer, N, n_eval) with N = N_star.  grid_mod's
+   ! omega_grid/rho_grid keep the last density() call's fields.
+   !--------------------------------------------------------------------
+   subroutine density_many(N, n_eval, omega, rho)
+   integer,    intent(IN)  :: N
+   integer,    intent(IN)  :: n_eval
+   real(long), intent(IN)  :: omega(:,:,:)
+   real(long), intent(OUT) :: rho(:,:,:)
+   real(c_double) :: phc(max(N_chain,1)), muc(max(N_chain,1))
+   real(c_double) :: phs(max(N_solvent,1)), mus(max(N_solvent,1))
+   if (size(omega,1) /= N_monomer .or. size(omega,2) /= N .or. size(omega,3) < n_eval) &
+      stop "density_many: omega is not (N_monomer, N, n_eval)"
+   if (size(rho,1) /= N_monomer .or. size(rho,2) /= N .or. size(rho,3) < n_eval) &
+      stop "density_many: rho is not (N_monomer, N, n_eval)"
+   if (n_eval <= 0) return
+   call sync_cell()
+   call sync_chi()
+   call chem_in(phc, muc, phs, mus)
+   call hip_check(c_pscf_density_many(hsys, int(n_eval, c_int), omega(:,:,1:n_eval), &
+                  rho(:,:,1:n_eval), c_null_ptr, c_null_ptr, phc, muc, phs, mus, 0_c_int), &
+                  'density_many')
+   if (ensemble == 1 .and. N_solvent > 0) phi_solvent(1:N_solvent) = phs(1:N_solvent)
+   end subroutine density_many
+
+   ! chemistry_mod's phi/mu as the library's in/out arrays
+   subroutine chem_in(phc, muc, phs, mus)
+   real(c_double), intent(OUT) :: phc(:), muc(:), phs(:), mus(:)
    phc = 0.0_long; muc = 0.0_long; phs = 0.0_long; mus = 0.0_long
    if (N_chain > 0) then
       phc(1:N_chain) = phi_chain(1:N_chain)
@@ -87,17 +141,7 @@ contains
       phs(1:N_solvent) = phi_solvent(1:N_solvent)
       mus(1:N_solvent) = mu_solvent(1:N_solvent)
    end if
-   call hip_check(c_pscf_density(hsys, omega, rho, q, qs, phc, muc, phs, mus), 'density')
-   if (present(qout)) qout(1:N_chain) = q(1:N_chain)
-   if (present(q_solvent)) q_solvent(1:N_solvent) = qs(1:N_solvent)
-   if (ensemble == 1) then
-      if (present(qout)) phi_chain(1:N_chain) = phc(1:N_chain)
-      if (present(q_solvent)) phi_solvent(1:N_solvent) = phs(1:N_solvent)
-   end if
-   if (sync_grid_globals) then
-      call hip_check(c_pscf_system_get_grids(hsys, omega_grid, rho_grid), 'density')
-   end if
-   end subroutine density
+   end subroutine chem_in
 
    !--------------------------------------------------------------------
    ! scf_stress(N, size_dGsq, dGsq)            src/scf_mod.fp.f90:451-550

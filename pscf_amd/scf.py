@@ -195,18 +195,39 @@ class SCFSystem:
         self.lib.call("pscf_system_startup", self.h, float(chain_step))
 
     # ------------------------------------------------------------ hot path
-    def density(self, omega):
-        """density(N, omega, rho, qout, q_solvent) (src/scf_mod.fp.f90:144-229).
+    def density(self, omega, with_q=True):
+        """density(N, omega, rho, qout, q_solvent) (src/scf_mod.fp.f90:144-229);
+        with_q False: without the optional qout/q_solvent.
         Returns (rho (N_monomer, N_star), q (N_chain,), q_solvent (N_solvent,))."""
         s = self.system
         om = _fortran2(omega)
         rho = np.empty_like(om)
         q = np.zeros(max(s.N_chain, 1))
         qs = np.zeros(max(len(s.solvents), 1))
-        self.lib.call("pscf_density", self.h, dptr(om), dptr(rho), dptr(q), dptr(qs),
+        self.lib.call("pscf_density", self.h, dptr(om), dptr(rho),
+                      dptr(q) if with_q else None, dptr(qs) if with_q else None,
                       dptr(self.phi_chain), dptr(self.mu_chain), dptr(self.phi_solvent),
                       dptr(self.mu_solvent))
         return rho.T.copy(), q[: s.N_chain].copy(), qs[: len(s.solvents)].copy()
+
+    def density_many(self, omegas, max_batch=0, with_q=True):
+        """density() at each of n omega fields (n, N_monomer, N_star) in one
+        launch set per max_batch evaluations (pscf_density_many); with_q False
+        calls it without qout/q_solvent, as the perturbed density() calls of
+        response_pd_omega (src/response_pd_mod.f90:114-119) do.  Returns rho
+        (n, N_monomer, N_star), q (n, N_chain), q_solvent (n, N_solvent)."""
+        s = self.system
+        om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).transpose(0, 2, 1))
+        n = om.shape[0]
+        rho = np.empty_like(om)
+        q = np.zeros((n, max(s.N_chain, 1)))
+        qs = np.zeros((n, max(len(s.solvents), 1)))
+        self.lib.call("pscf_density_many", self.h, n, dptr(om), dptr(rho),
+                      dptr(q) if with_q else None, dptr(qs) if with_q else None,
+                      dptr(self.phi_chain), dptr(self.mu_chain), dptr(self.phi_solvent),
+                      dptr(self.mu_solvent), int(max_batch))
+        return (rho.transpose(0, 2, 1).copy(), q[:, : s.N_chain].copy(),
+                qs[:, : len(s.solvents)].copy())
 
     def scf_stress(self, dGsq):
         """scf_stress(N, size_dGsq, dGsq) (src/scf_mod.fp.f90:451-550)."""
