@@ -278,6 +278,23 @@ int pscf_density_grid_dev(int64_t sys, const double* omega_grid, double* rho_gri
 int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const double* dL_dQ,
                              double* grad_omega_grid, int n_cell_param, const double* dGG,
                              double* dL_dcell, double* dL_dlength, double* dL_dphi);
+/* Reverse mode of several systems at once: systems on one grid (any
+ * chemistry, one checkpoint stride) run their adjoint sweeps -- and the
+ * checkpoint rounds that recompute their contour segments -- in one launch
+ * set per contour step.  Each system's results equal its
+ * pscf_density_adjoint_dev call's.  The group does not own the systems:
+ * destroy it before them. */
+int pscf_adjoint_group_create(int n, const int64_t* systems, int64_t* group);
+int pscf_adjoint_group_destroy(int64_t group);
+/* Arguments are arrays over the group's systems, entry i as the
+ * corresponding argument of pscf_density_adjoint_dev for system i (after
+ * its last density() call).  dL_dQ, n_cell_param, dGG, dL_dcell, dL_dlength
+ * and dL_dphi may be NULL, and so may their entries. */
+int pscf_adjoint_group_run(int64_t group, const double* const* lam_rho_grid,
+                           const double* const* dL_dQ, double* const* grad_omega_grid,
+                           const int* n_cell_param, const double* const* dGG,
+                           double* const* dL_dcell, double* const* dL_dlength,
+                           double* const* dL_dphi);
 /* basis_to_kgrid + ifft of nf star fields, device basis (nf, N_star) ->
  * device grids (M, nf)                        src/scf_mod.fp.f90:182-185 */
 int pscf_basis_to_grid_dev(int64_t sys, const double* basis, int nf, double* grid);

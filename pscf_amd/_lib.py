@@ -14,6 +14,7 @@ DEFAULT_LIB = os.path.join(HERE, "libpscf_hip.so")
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 _c_i64_p = ctypes.POINTER(ctypes.c_int64)
+_c_vp_p = ctypes.POINTER(ctypes.c_void_p)
 
 # name -> argtypes (every function returns int status)
 SIGNATURES = {
@@ -57,6 +58,10 @@ SIGNATURES = {
                               _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pscf_density_adjoint_dev": [ctypes.c_int64, ctypes.c_void_p, _c_dbl_p, ctypes.c_void_p,
                                  ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pscf_adjoint_group_create": [ctypes.c_int, _c_i64_p, _c_i64_p],
+    "pscf_adjoint_group_destroy": [ctypes.c_int64],
+    "pscf_adjoint_group_run": [ctypes.c_int64, _c_vp_p, _c_vp_p, _c_vp_p, _c_int_p, _c_vp_p,
+                               _c_vp_p, _c_vp_p, _c_vp_p],
     "pscf_basis_to_grid_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "pscf_grid_to_basis_dev": [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                ctypes.c_void_p],

@@ -77,71 +77,168 @@ def set_state(S, cell_param=None, lengths=None):
             S.startup(sysm.chain_step)
 
 
+def _forward(S, omega_grid, cell_param, lengths, phi):
+    """density() of S on grids (pscf_density_grid_dev) at the given state;
+    returns rho_grid, Q and the forward's generation stamp."""
+    set_state(S, cell_param.detach().cpu().numpy(), lengths.detach().cpu().numpy())
+    S.phi_chain[: len(phi)] = phi.detach().cpu().numpy()
+    s = S.system
+    om = omega_grid.detach().to(torch.float64).contiguous()
+    assert om.shape == (s.N_monomer,) + S.shape, (om.shape, S.shape)
+    rho = torch.empty_like(om)
+    q = np.zeros(max(s.N_chain, 1))
+    qs = np.zeros(max(len(s.solvents), 1))
+    _sync(om)
+    S.lib.call("pscf_density_grid_dev", S.h, _vp(om), _vp(rho), dptr(q), dptr(qs),
+               dptr(S.phi_chain), dptr(S.mu_chain), dptr(S.phi_solvent), dptr(S.mu_solvent))
+    S._density_gen = getattr(S, "_density_gen", 0) + 1
+    return rho, torch.tensor(q[: s.N_chain], dtype=torch.float64, device=om.device), S._density_gen
+
+
+class _Backward:
+    """Arguments and results of one system's adjoint call."""
+
+    def __init__(self, S, gen, g_rho, g_Q, want_cell, want_len, want_phi):
+        if getattr(S, "_density_gen", 0) != gen:
+            raise RuntimeError("the system evaluated density() again after this forward; "
+                               "its contour slabs are gone (call backward first)")
+        s = S.system
+        dev = torch.device(S.lib.tensor_device)
+        self.S, self.want = S, (want_cell, want_len, want_phi)
+        self.lam = (torch.zeros((s.N_monomer,) + S.shape, dtype=torch.float64, device=dev)
+                    if g_rho is None else g_rho.detach().to(torch.float64).contiguous())
+        self.dQ = (np.zeros(max(s.N_chain, 1)) if g_Q is None
+                   else np.ascontiguousarray(g_Q.detach().cpu().numpy(), dtype=np.float64))
+        self.grad = torch.empty_like(self.lam)
+        self.ncp = len(s.cell_param)
+        self.dcell = np.zeros(max(self.ncp, 1))
+        self.dlen = np.zeros(sum(len(c) for c in s.chains))
+        self.dphi = np.zeros(max(s.N_chain, 1))
+        self.dgg = None
+        if want_cell:
+            _, _, dGG = unit_cell(S.lib, s.dim, s.crystal_system, s.cell_param)
+            self.dgg = np.ascontiguousarray(np.asarray(dGG[:, :, : self.ncp]).transpose(2, 1, 0))
+        _sync(self.lam)
+
+    def results(self):
+        want_cell, want_len, want_phi = self.want
+        nc = self.S.system.N_chain
+        return (self.grad,
+                torch.tensor(self.dcell[: self.ncp], dtype=torch.float64) if want_cell else None,
+                torch.tensor(self.dlen, dtype=torch.float64) if want_len else None,
+                torch.tensor(self.dphi[:nc], dtype=torch.float64) if want_phi else None)
+
+
 class DensityFunction(torch.autograd.Function):
     """(omega_grid, cell_param, block_lengths, phi_chain) -> (rho_grid, Q)."""
 
     @staticmethod
     def forward(ctx, omega_grid, cell_param, lengths, phi, S):
-        set_state(S, cell_param.detach().cpu().numpy(), lengths.detach().cpu().numpy())
-        S.phi_chain[: len(phi)] = phi.detach().cpu().numpy()
-        s = S.system
-        om = omega_grid.detach().to(torch.float64).contiguous()
-        assert om.shape == (s.N_monomer,) + S.shape, (om.shape, S.shape)
-        rho = torch.empty_like(om)
-        q = np.zeros(max(s.N_chain, 1))
-        qs = np.zeros(max(len(s.solvents), 1))
-        _sync(om)
-        S.lib.call("pscf_density_grid_dev", S.h, _vp(om), _vp(rho), dptr(q), dptr(qs),
-                   dptr(S.phi_chain), dptr(S.mu_chain), dptr(S.phi_solvent), dptr(S.mu_solvent))
-        S._density_gen = getattr(S, "_density_gen", 0) + 1
-        ctx.S = S
-        ctx.gen = S._density_gen
-        ctx.ncp = len(s.cell_param)
-        return rho, torch.tensor(q[: s.N_chain], dtype=torch.float64, device=om.device)
+        rho, Q, gen = _forward(S, omega_grid, cell_param, lengths, phi)
+        ctx.S, ctx.gen = S, gen
+        return rho, Q
 
     @staticmethod
     def backward(ctx, g_rho, g_Q):
         S = ctx.S
-        if getattr(S, "_density_gen", 0) != ctx.gen:
-            raise RuntimeError("the system evaluated density() again after this forward; "
-                               "its contour slabs are gone (call backward first)")
-        s = S.system
-        dev = torch.device(S.lib.tensor_device)
-        lam = (torch.zeros((s.N_monomer,) + S.shape, dtype=torch.float64, device=dev)
-               if g_rho is None else g_rho.detach().to(torch.float64).contiguous())
-        dQ = (np.zeros(max(s.N_chain, 1)) if g_Q is None
-              else np.ascontiguousarray(g_Q.detach().cpu().numpy(), dtype=np.float64))
-        grad = torch.empty_like(lam)
-        want_cell, want_len, want_phi = ctx.needs_input_grad[1:4]
-        ncp = ctx.ncp
-        dcell = np.zeros(max(ncp, 1))
-        dlen = np.zeros(sum(len(c) for c in s.chains))
-        dphi = np.zeros(max(s.N_chain, 1))
-        dgg_arg = None
-        if want_cell:
-            _, _, dGG = unit_cell(S.lib, s.dim, s.crystal_system, s.cell_param)
-            dgg_arg = np.ascontiguousarray(np.asarray(dGG[:, :, :ncp]).transpose(2, 1, 0))
-        _sync(lam)
-        S.lib.call("pscf_density_adjoint_dev", S.h, _vp(lam), dptr(dQ), _vp(grad), ncp,
-                   dptr(dgg_arg) if want_cell else None, dptr(dcell) if want_cell else None,
-                   dptr(dlen) if want_len else None, dptr(dphi) if want_phi else None)
-        return (grad,
-                torch.tensor(dcell[:ncp], dtype=torch.float64) if want_cell else None,
-                torch.tensor(dlen, dtype=torch.float64) if want_len else None,
-                torch.tensor(dphi[: s.N_chain], dtype=torch.float64) if want_phi else None,
-                None)
+        b = _Backward(S, ctx.gen, g_rho, g_Q, *ctx.needs_input_grad[1:4])
+        want_cell, want_len, want_phi = b.want
+        S.lib.call("pscf_density_adjoint_dev", S.h, _vp(b.lam), dptr(b.dQ), _vp(b.grad), b.ncp,
+                   dptr(b.dgg) if want_cell else None, dptr(b.dcell) if want_cell else None,
+                   dptr(b.dlen) if want_len else None, dptr(b.dphi) if want_phi else None)
+        return b.results() + (None,)
 
 
-def density(S, omega_grid, cell_param=None, lengths=None, phi=None):
-    """rho_grid, Q of system S (an SCFSystem) at omega_grid; cell_param,
-    lengths (block lengths, chain-major) and phi (chain volume fractions),
-    float64 tensors, optional, make those differentiable inputs."""
+def _ptrs(items):
+    return (ctypes.c_void_p * len(items))(*items)
+
+
+class _GroupDensity(torch.autograd.Function):
+    """density() of every system of a Group; one grouped backward."""
+
+    @staticmethod
+    def forward(ctx, group, *args):
+        outs, gens = [], []
+        for i, S in enumerate(group.systems):
+            rho, Q, gen = _forward(S, *args[4 * i: 4 * i + 4])
+            outs += [rho, Q]
+            gens.append(gen)
+        ctx.group, ctx.gens = group, gens
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        G = ctx.group
+        need = ctx.needs_input_grad[1:]
+        bs = [_Backward(S, ctx.gens[i], grads[2 * i], grads[2 * i + 1], *need[4 * i + 1: 4 * i + 4])
+              for i, S in enumerate(G.systems)]
+
+        def arr(f):
+            return _ptrs([f(b) for b in bs])
+
+        ncp = (ctypes.c_int * len(bs))(*[b.ncp for b in bs])
+        G.lib.call("pscf_adjoint_group_run", G.h,
+                   arr(lambda b: b.lam.data_ptr()), arr(lambda b: b.dQ.ctypes.data),
+                   arr(lambda b: b.grad.data_ptr()), ncp,
+                   arr(lambda b: b.dgg.ctypes.data if b.want[0] else None),
+                   arr(lambda b: b.dcell.ctypes.data if b.want[0] else None),
+                   arr(lambda b: b.dlen.ctypes.data if b.want[1] else None),
+                   arr(lambda b: b.dphi.ctypes.data if b.want[2] else None))
+        out = [None]
+        for b in bs:
+            out += list(b.results())
+        return tuple(out)
+
+
+class Group:
+    """Systems on one grid (SCFSystems; any chemistry, one checkpoint stride)
+    whose backward passes run as one set of launches
+    (pscf_adjoint_group_*).  density() returns, per system, what
+    adjoint.density() returns; a loss built from them backpropagates through
+    all systems' adjoint sweeps at once."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        self.lib = self.systems[0].lib
+        n = len(self.systems)
+        arr = (ctypes.c_int64 * n)(*[S.h for S in self.systems])
+        h = ctypes.c_int64(0)
+        self.lib.call("pscf_adjoint_group_create", n, arr, ctypes.byref(h))
+        self.h = h.value
+
+    def __del__(self):
+        if getattr(self, "h", 0):
+            self.lib.dll.pscf_adjoint_group_destroy(self.h)
+            self.h = 0
+
+    def density(self, omega_grids, cell_params=None, lengths=None, phis=None):
+        args = []
+        for i, S in enumerate(self.systems):
+            cp = cell_params[i] if cell_params is not None else None
+            L = lengths[i] if lengths is not None else None
+            ph = phis[i] if phis is not None else None
+            args += list(_defaults(S, cp, L, ph))
+            args[-4:-3] = [omega_grids[i]]
+        outs = _GroupDensity.apply(self, *args)
+        return [(outs[2 * i], outs[2 * i + 1]) for i in range(len(self.systems))]
+
+
+def _defaults(S, cell_param, lengths, phi):
+    """(None, cell, lengths, phi) with the system's values for those not given."""
     if cell_param is None:
         cell_param = torch.tensor(np.asarray(S.system.cell_param, dtype=np.float64))
     if lengths is None:
         lengths = block_lengths(S.system)
     if phi is None:
         phi = torch.tensor(np.asarray(S.phi_chain[: S.system.N_chain], dtype=np.float64))
+    return None, cell_param, lengths, phi
+
+
+def density(S, omega_grid, cell_param=None, lengths=None, phi=None):
+    """rho_grid, Q of system S (an SCFSystem) at omega_grid; cell_param,
+    lengths (block lengths, chain-major) and phi (chain volume fractions),
+    float64 tensors, optional, make those differentiable inputs."""
+    _, cell_param, lengths, phi = _defaults(S, cell_param, lengths, phi)
     return DensityFunction.apply(omega_grid, cell_param, lengths, phi, S)
 
 

@@ -3,6 +3,7 @@
 // kernels.hpp / kernels_aux.hpp.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -326,6 +327,7 @@ struct CkPlan {
 // (adjoint of q, recomputed q, adjoint of q-dagger, recomputed q-dagger),
 // their work and kept-field buffers, and the gradient accumulators.
 struct AdjState {
+  uint64_t id = 0;         // distinct for every AdjState made
   int ntask = 0, npair = 0;
   bool with_k = false;
   DevBuf<double> keep;     // per task: U, V, W (m each); adjoint tasks also 2 m of kept inputs
@@ -344,7 +346,8 @@ struct AdjState {
   DevBuf<StepTask> ck_rec, ck_adj;
   DevBuf<double2> rec_abc;
   DevBuf<double> rec_q1;
-  std::vector<StepTask> h_tasks;
+  std::vector<StepTask> h_tasks, h_ck_rec, h_ck_adj;
+  std::vector<int2> h_pairs;
   std::vector<size_t> blk_off;   // per chain: first block in gblk/kblk
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1184,6 +1187,8 @@ void adjoint_setup(System& S, bool with_k) {
   if (S.adj && S.adj->with_k == with_k) return;
   S.adj.reset();
   auto A = std::make_unique<AdjState>();
+  static std::atomic<uint64_t> next_id{1};
+  A->id = next_id++;
   const Grid& g = S.g;
   A->with_k = with_k;
   A->ntask = 4 * S.nchain;
@@ -1319,33 +1324,48 @@ void adjoint_setup(System& S, bool with_k) {
       }
     A->ck_rec.upload(rec.data(), rec.size(), S.st.s);
     A->ck_adj.upload(adj.data(), adj.size(), S.st.s);
+    A->h_ck_rec = rec;
+    A->h_ck_adj = adj;
   }
   S.st.sync();
   A->h_tasks = ht;
+  A->h_pairs = hp;
   S.adj = std::move(A);
 }
 
 // the adjoint sweeps with their gradient kernels (launch order as run_sweeps).
 // Checkpointed storage: step j runs with the tasks of round j / K, whose
 // segments the recomputation sweeps fill just before the round's first step.
-void run_adj_sweeps(const System& S, hipStream_t s) {
-  const Grid& g = S.g;
-  const AdjState& A = *S.adj;
-  const int nt = A.ntask, max_ns = S.max_ns;
+// One launch set of adjoint sweeps: the tasks and pairs of one or more
+// systems on one grid, with the checkpoint rounds (K > 0) of all of them.
+struct AdjSweeps {
+  const StepTask* tasks = nullptr;
+  int ntask = 0;
+  const int2* pairs = nullptr;
+  int npair = 0;
+  int max_ns = 0;
+  bool with_k = false;
+  int K = 0, nround = 0;
+  const int* len = nullptr;              // per round: longest segment
+  const StepTask* ck_rec = nullptr;      // per round: ntask recomputation tasks
+  const StepTask* ck_adj = nullptr;      // per round: ntask adjoint tasks
+};
+
+void run_adj_sweeps(const Grid& g, const AdjSweeps& W, hipStream_t s) {
+  const int nt = W.ntask, max_ns = W.max_ns;
   const int gb = grid_blocks(g.m), gk = grid_blocks(g.mk);
-  const int K = S.ck_stride;
-  const StepTask* tasks = A.tasks.p;
+  const int K = W.K;
+  const StepTask* tasks = W.tasks;
   for (int j = 0; j <= max_ns; ++j) {
     if (K > 0) {
-      const int r = std::min(j / K, S.ck.nround - 1);
-      if (j % K == 0 && j / K < S.ck.nround)
-        run_sweeps(g, A.ck_rec.p + (size_t)r * nt, nt, S.ck.len[r], s);
-      tasks = A.ck_adj.p + (size_t)r * nt;
+      const int r = std::min(j / K, W.nround - 1);
+      if (j % K == 0 && j / K < W.nround)
+        run_sweeps(g, W.ck_rec + (size_t)r * nt, nt, W.len[r], s);
+      tasks = W.ck_adj + (size_t)r * nt;
     }
     launch_step_a(g, tasks, nt, j, s);
     if (j >= 1)
-      hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, tasks, A.pairs.p, A.npair,
-                         j - 1, g.m);
+      hipLaunchKernelGGL(k_adj_grad, dim3(gb), dim3(256), 0, s, tasks, W.pairs, W.npair, j - 1, g.m);
     if (j == max_ns) break;
     hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 2), dim3(g.zs->nt_step), 0, s, tasks, j, g.n2, 0,
                        g.tw3);
@@ -1353,11 +1373,31 @@ void run_adj_sweeps(const System& S, hipStream_t s) {
                        g.tw2);
     hipLaunchKernelGGL(g.zs->z, dim3(g.n2 * g.zs->zsplit, nt, 1), dim3(g.zs->nt_step), 0, s, tasks, j, g.n2, 1,
                        g.tw3);
-    if (A.with_k)
-      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk), dim3(256), 0, s, tasks, A.pairs.p, A.npair, j,
-                         g.n1, g.mk, (double)g.m);
+    if (W.with_k)
+      hipLaunchKernelGGL(k_adj_kgrad, dim3(gk), dim3(256), 0, s, tasks, W.pairs, W.npair, j, g.n1,
+                         g.mk, (double)g.m);
   }
   HIPCHK(hipGetLastError());
+}
+
+// the adjoint sweeps of one system
+void run_adj_sweeps(const System& S, hipStream_t s) {
+  const AdjState& A = *S.adj;
+  AdjSweeps W;
+  W.tasks = A.tasks.p;
+  W.ntask = A.ntask;
+  W.pairs = A.pairs.p;
+  W.npair = A.npair;
+  W.max_ns = S.max_ns;
+  W.with_k = A.with_k;
+  W.K = S.ck_stride;
+  if (W.K > 0) {
+    W.nround = S.ck.nround;
+    W.len = S.ck.len.data();
+    W.ck_rec = A.ck_rec.p;
+    W.ck_adj = A.ck_adj.p;
+  }
+  run_adj_sweeps(S.g, W, s);
 }
 
 // reduce np device partials to one host value (stream synchronised)
@@ -1376,21 +1416,30 @@ double host_sum(System& S, const double* part, int np) {
 // (block step count fixed; the chain length in the prefactor is the sum of
 // the block lengths); with dphi dL/d phi_chain (canonical ensemble: rho of a
 // chain is proportional to its phi; 0 in the grand ensemble).
-void density_adjoint(System& S, const double* lam, const double* dQ, double* grad, int ncp,
-                     const double* dgg, double* dcell, double* dlen, double* dphi) {
+// Host scalars of one adjoint call: the Simpson sums T_b and the density
+// prefactor c of every chain.
+struct AdjCall {
+  std::vector<std::vector<double>> T;
+  std::vector<double> coef;
+};
+
+// Before the sweeps: adjoint state (with |k|^2 gradients when with_k), the
+// sums T_b = sum_r lam_b rho-integrand of every block (for Gam and d/d ds),
+// the sources' coefficients, cleared gradient accumulators.
+void adjoint_pre(System& S, const double* lam, const double* dQ, bool with_k, AdjCall& C) {
   REQUIRE(S.have_density, "the adjoint requires a previous density()");
   REQUIRE(S.nsolv == 0, "the adjoint does not support solvent species");
-  const bool with_k = dgg != nullptr || dlen != nullptr;
   adjoint_setup(S, with_k);
   AdjState& A = *S.adj;
   const Grid& g = S.g;
   hipStream_t s = S.st.s;
   const int nb = ADJ_NB;
-  REQUIRE(ncp <= 6, "at most 6 cell parameters");
   double* part = A.part.p;
-  // Simpson sums T_b = sum_r lam_b rho-integrand of every block (for Gam and d/d ds)
-  std::vector<std::vector<double>> T(S.nchain);
-  std::vector<double> coef(S.nchain), gam(S.nchain);
+  auto& T = C.T;
+  auto& coef = C.coef;
+  T.assign(S.nchain, {});
+  coef.assign(S.nchain, 0.0);
+  std::vector<double> gam(S.nchain);
   std::vector<SimpsonArgs> sa(S.nchain);
   for (int c = 0; c < S.nchain; ++c) {
     ChainDev& ch = *S.chains[c];
@@ -1438,18 +1487,20 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
   if (with_k)
     hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)(nblk * g.mk))), dim3(NTA), 0, s, A.kblk.p,
                        (long)(nblk * g.mk), 0.0);
-  // the sweeps (captured once as a graph)
-  if (S.use_graph) {
-    if (!A.exec) {
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      run_adj_sweeps(S, s);
-      HIPCHK(hipStreamEndCapture(s, &A.graph));
-      HIPCHK(hipGraphInstantiate(&A.exec, A.graph, nullptr, nullptr, 0));
-    }
-    HIPCHK(hipGraphLaunch(A.exec, s));
-  } else {
-    run_adj_sweeps(S, s);
-  }
+}
+
+// After the sweeps: the gradients from the accumulators (see density_adjoint)
+void adjoint_post(System& S, const AdjCall& C, double* grad, int ncp, const double* dgg,
+                  double* dcell, double* dlen, double* dphi) {
+  AdjState& A = *S.adj;
+  const Grid& g = S.g;
+  hipStream_t s = S.st.s;
+  const int nb = ADJ_NB;
+  double* part = A.part.p;
+  const auto& T = C.T;
+  const auto& coef = C.coef;
+  size_t nblk = 0;
+  for (int c = 0; c < S.nchain; ++c) nblk += S.chains[c]->nblk;
   // omega gradient per monomer: sum of its blocks
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks((long)S.nmon * g.m)), dim3(NTA), 0, s, grad,
                      (long)S.nmon * g.m, 0.0);
@@ -1512,6 +1563,163 @@ void density_adjoint(System& S, const double* lam, const double* dQ, double* gra
     }
   }
   S.st.sync();
+}
+
+void density_adjoint(System& S, const double* lam, const double* dQ, double* grad, int ncp,
+                     const double* dgg, double* dcell, double* dlen, double* dphi) {
+  REQUIRE(ncp <= 6, "at most 6 cell parameters");
+  const bool with_k = dgg != nullptr || dlen != nullptr;
+  AdjCall C;
+  adjoint_pre(S, lam, dQ, with_k, C);
+  AdjState& A = *S.adj;
+  hipStream_t s = S.st.s;
+  // the sweeps (captured once as a graph)
+  if (S.use_graph) {
+    if (!A.exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_adj_sweeps(S, s);
+      HIPCHK(hipStreamEndCapture(s, &A.graph));
+      HIPCHK(hipGraphInstantiate(&A.exec, A.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(A.exec, s));
+  } else {
+    run_adj_sweeps(S, s);
+  }
+  adjoint_post(S, C, grad, ncp, dgg, dcell, dlen, dphi);
+}
+
+// ---- reverse mode of several systems at once
+//
+// Systems on one grid (any chemistry, one checkpoint stride) run their
+// adjoint sweeps, and the checkpoint rounds that recompute their segments,
+// in one launch set per contour step, captured once as a HIP graph.  Each
+// system's sources, reductions and gradients are its own (adjoint_pre/post);
+// every system's tasks run the kernels of its lone adjoint, so the results
+// equal the single calls'.
+struct AdjGroup {
+  std::vector<System*> sys;
+  std::vector<uint64_t> ids;     // AdjState ids the launch set was built from
+  bool with_k = false;
+  Stream st;
+  DevBuf<StepTask> tasks, ck_rec, ck_adj;
+  DevBuf<int2> pairs;
+  std::vector<int> len;
+  AdjSweeps W;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~AdjGroup() { drop_graph(); }
+};
+
+void build_adj_group(AdjGroup& G) {
+  const System& S0 = *G.sys[0];
+  const int K = S0.ck_stride;
+  int ntot = 0, max_ns = 0, nround = 0;
+  for (System* S : G.sys) {
+    REQUIRE(S->g.n1 == S0.g.n1 && S->g.n2 == S0.g.n2 && S->g.n3 == S0.g.n3,
+            "adjoint group: systems on different grids");
+    REQUIRE(S->ck_stride == K, "adjoint group: systems with different checkpoint strides");
+    ntot += S->adj->ntask;
+    max_ns = std::max(max_ns, S->max_ns);
+    if (K > 0) nround = std::max(nround, S->ck.nround);
+  }
+  std::vector<StepTask> ht;
+  std::vector<int2> hp;
+  std::vector<int> off;
+  for (System* S : G.sys) {
+    const AdjState& A = *S->adj;
+    const int o = (int)ht.size();
+    off.push_back(o);
+    ht.insert(ht.end(), A.h_tasks.begin(), A.h_tasks.end());
+    for (const int2& p : A.h_pairs) hp.push_back(make_int2(p.x + o, p.y + o));
+  }
+  G.tasks.upload(ht.data(), ht.size(), G.st.s);
+  G.pairs.upload(hp.data(), hp.size(), G.st.s);
+  G.W = AdjSweeps();
+  G.W.tasks = G.tasks.p;
+  G.W.ntask = ntot;
+  G.W.pairs = G.pairs.p;
+  G.W.npair = (int)hp.size();
+  G.W.max_ns = max_ns;
+  G.W.with_k = G.with_k;
+  G.W.K = K;
+  if (K > 0) {
+    // round r of a system past its own rounds: recomputation skipped, adjoint
+    // tasks of its last round (whose windows end its sweeps)
+    std::vector<StepTask> rec((size_t)nround * ntot), adj((size_t)nround * ntot);
+    G.len.assign(nround, 0);
+    for (int r = 0; r < nround; ++r)
+      for (size_t i = 0; i < G.sys.size(); ++i) {
+        const System& S = *G.sys[i];
+        const AdjState& A = *S.adj;
+        const int nt = A.ntask, ri = std::min(r, S.ck.nround - 1);
+        for (int k = 0; k < nt; ++k) {
+          StepTask tr = A.h_ck_rec[(size_t)ri * nt + k];
+          if (r >= S.ck.nround) tr.skip = 1;
+          rec[(size_t)r * ntot + off[i] + k] = tr;
+          adj[(size_t)r * ntot + off[i] + k] = A.h_ck_adj[(size_t)ri * nt + k];
+        }
+        if (r < S.ck.nround) G.len[r] = std::max(G.len[r], S.ck.len[r]);
+      }
+    G.ck_rec.upload(rec.data(), rec.size(), G.st.s);
+    G.ck_adj.upload(adj.data(), adj.size(), G.st.s);
+    G.W.nround = nround;
+    G.W.len = G.len.data();
+    G.W.ck_rec = G.ck_rec.p;
+    G.W.ck_adj = G.ck_adj.p;
+  }
+  G.st.sync();
+  G.ids.clear();
+  for (System* S : G.sys) G.ids.push_back(S->adj->id);
+  G.drop_graph();
+}
+
+// per system i: lam[i], dQ[i] (may be null) in; grad[i] out; with dgg[i] the
+// cell gradient, with dlen[i] the block-length gradient, with dphi[i] the
+// phi_chain gradient, as density_adjoint (the pointer arrays may be null)
+void adjoint_group(AdjGroup& G, const double* const* lam, const double* const* dQ,
+                   double* const* grad, const int* ncp, const double* const* dgg,
+                   double* const* dcell, double* const* dlen, double* const* dphi) {
+  const size_t n = G.sys.size();
+  bool with_k = false;
+  for (size_t i = 0; i < n; ++i) {
+    REQUIRE(lam && lam[i] && grad && grad[i], "null grid pointer");
+    const int nc = ncp ? ncp[i] : 0;
+    REQUIRE(nc >= 0 && nc <= 6, "at most 6 cell parameters");
+    REQUIRE(!(dgg && dgg[i]) || (dcell && dcell[i]), "dGG given without an output for the cell gradient");
+    with_k = with_k || (dgg && dgg[i]) || (dlen && dlen[i]);
+  }
+  std::vector<AdjCall> C(n);
+  for (size_t i = 0; i < n; ++i)
+    adjoint_pre(*G.sys[i], lam[i], dQ ? dQ[i] : nullptr, with_k, C[i]);
+  for (System* S : G.sys) S->st.sync();
+  bool same = G.ids.size() == n && G.with_k == with_k;
+  for (size_t i = 0; same && i < n; ++i) same = G.sys[i]->adj->id == G.ids[i];
+  if (!same) {
+    G.with_k = with_k;
+    build_adj_group(G);
+  }
+  hipStream_t s = G.st.s;
+  if (G.sys[0]->use_graph) {
+    if (!G.exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_adj_sweeps(G.sys[0]->g, G.W, s);
+      HIPCHK(hipStreamEndCapture(s, &G.graph));
+      HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(G.exec, s));
+  } else {
+    run_adj_sweeps(G.sys[0]->g, G.W, s);
+  }
+  G.st.sync();
+  for (size_t i = 0; i < n; ++i)
+    adjoint_post(*G.sys[i], C[i], grad[i], ncp ? ncp[i] : 0, dgg ? dgg[i] : nullptr,
+                 dcell ? dcell[i] : nullptr, dlen ? dlen[i] : nullptr, dphi ? dphi[i] : nullptr);
 }
 
 // ---- scf_stress (src/scf_mod.fp.f90:451-550)
@@ -2350,6 +2558,34 @@ int pscf_density_adjoint_dev(int64_t sys, const double* lam_rho_grid, const doub
     REQUIRE(!dGG || dL_dcell, "dGG given without an output for the cell gradient");
     density_adjoint(*from_handle<System>(sys, "system"), lam_rho_grid, dL_dQ, grad_omega_grid,
                     n_cell_param, dGG, dL_dcell, dL_dlength, dL_dphi);
+  });
+}
+
+int pscf_adjoint_group_create(int n, const int64_t* systems, int64_t* group) {
+  return guarded([&] {
+    REQUIRE(n >= 1 && systems && group, "invalid arguments");
+    auto G = std::make_unique<AdjGroup>();
+    for (int i = 0; i < n; ++i) {
+      System* S = from_handle<System>(systems[i], "system");
+      for (System* o : G->sys) REQUIRE(o != S, "adjoint group: a system appears twice");
+      G->sys.push_back(S);
+    }
+    *group = reinterpret_cast<int64_t>(G.release());
+  });
+}
+
+int pscf_adjoint_group_destroy(int64_t group) {
+  return guarded([&] { delete from_handle<AdjGroup>(group, "adjoint group"); });
+}
+
+int pscf_adjoint_group_run(int64_t group, const double* const* lam_rho_grid,
+                           const double* const* dL_dQ, double* const* grad_omega_grid,
+                           const int* n_cell_param, const double* const* dGG,
+                           double* const* dL_dcell, double* const* dL_dlength,
+                           double* const* dL_dphi) {
+  return guarded([&] {
+    adjoint_group(*from_handle<AdjGroup>(group, "adjoint group"), lam_rho_grid, dL_dQ,
+                  grad_omega_grid, n_cell_param, dGG, dL_dcell, dL_dlength, dL_dphi);
   });
 }
 

@@ -366,3 +366,86 @@ def test_gyroid64_loss_gradients_gpu():
     assert abs(cp.grad.item() - fd_a) <= FD_TOL * abs(fd_a), (cp.grad.item(), fd_a)
     assert abs(fA.grad.item() - fd_f) <= FD_TOL * abs(fd_f), (fA.grad.item(), fd_f)
     assert abs(ad_w - fd2_w) <= 1e-5 * abs(fd2_w), (ad_w, fd2_w)
+
+
+def _group_systems(backend, stride):
+    """Two systems on the BCC 8^3 grid and star basis: the diblock, and an
+    ABA triblock + A homopolymer blend at a finer contour step (different
+    step counts, so with checkpoints different numbers of rounds)."""
+    lib = get_lib(backend)
+    g = load_golden("bcc8")
+    a = copy.deepcopy(cases()["bcc8"][0])
+    b = System(kuhn=a.kuhn, chains=[[(1, 1.5), (2, 3.5), (1, 2.0)], [(1, 3.0)]],
+               phi_chain=[0.8, 0.2], chi=a.chi, dim=a.dim, crystal_system=a.crystal_system,
+               cell_param=a.cell_param, ngrid=a.ngrid, chain_step=0.4, group_name=a.group_name)
+    out = []
+    for s in (a, b):
+        S = batch.make_system(s, g, lib=lib)
+        if stride:
+            S.set_checkpoint(stride)
+        out.append((S, s))
+    return out, g, lib.tensor_device
+
+
+def _group_inputs(sys_list, g, dev, seed):
+    rng = np.random.default_rng(seed)
+    ins = []
+    for S, s in sys_list:
+        om = adjoint.stars_to_grid(S, torch.tensor(g["omega"], dtype=torch.float64, device=dev))
+        om = (om.detach() + torch.tensor(0.05 * rng.standard_normal(om.shape), device=dev))
+        tgt = torch.tensor(rng.normal(0.3, 0.05, om.shape[1:]), device=dev)
+        ins.append((om, tgt, np.array(s.cell_param, dtype=np.float64),
+                    adjoint.block_lengths(s).numpy(), np.array(s.phi_chain, dtype=np.float64)))
+    return ins
+
+
+def _group_loss(rho, Q, tgt, nchain):
+    loss = ((rho[0] - tgt) ** 2).sum()
+    for c in range(nchain):
+        loss = loss + 0.3 * (c + 1) * torch.log(Q[c])
+    return loss
+
+
+def _leaves(om, cp, L, ph):
+    return [torch.tensor(x, dtype=torch.float64).requires_grad_(True) if not torch.is_tensor(x)
+            else x.clone().requires_grad_(True) for x in (om, cp, L, ph)]
+
+
+@pytest.mark.parametrize("stride", [pytest.param(0, marks=pytest.mark.gpu), 3])
+def test_group_adjoint_equals_single(backend, stride):
+    """adjoint.Group: the backward passes of two systems (one and two chains,
+    20 and 18 contour steps) in one launch set give each system's own
+    gradients bit for bit, with full and checkpointed storage (7 and 6
+    rounds); on the GPU the group is also reused for new fields and rebuilt
+    when fewer gradients are asked for."""
+    singles, g, dev = _group_systems(backend, stride)
+    grouped, _, _ = _group_systems(backend, stride)
+    G = adjoint.Group([S for S, _ in grouped])
+    rounds = ((5, True), (6, True), (7, False)) if backend == "gpu" else ((5, True),)
+    for seed, full in rounds:
+        ins = _group_inputs(singles, g, dev, seed)
+        want = []
+        for (S, s), (om, tgt, cp, L, ph) in zip(singles, ins):
+            xs = _leaves(om, cp, L, ph)
+            if not full:
+                xs[1].requires_grad_(False)
+                xs[2].requires_grad_(False)
+            rho, Q = adjoint.density(S, *xs)
+            _group_loss(rho, Q, tgt, s.N_chain).backward()
+            want.append([x.grad for x in xs])
+        xss = [_leaves(om, cp, L, ph) for (om, tgt, cp, L, ph) in ins]
+        if not full:
+            for xs in xss:
+                xs[1].requires_grad_(False)
+                xs[2].requires_grad_(False)
+        outs = G.density([xs[0] for xs in xss], [xs[1] for xs in xss], [xs[2] for xs in xss],
+                         [xs[3] for xs in xss])
+        loss = sum(_group_loss(rho, Q, ins[i][1], grouped[i][1].N_chain)
+                   for i, (rho, Q) in enumerate(outs))
+        loss.backward()
+        for i, xs in enumerate(xss):
+            for k, (a, b) in enumerate(zip([x.grad for x in xs], want[i])):
+                if b is None:
+                    assert a is None, (seed, i, k)
+                else:
+                    assert torch.equal(a.cpu(), b.cpu()), (seed, i, k, (a - b).abs().max())
