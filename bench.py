@@ -282,6 +282,112 @@ def jacobian_leg(L, cuts=(17, 33), reps=3):
             "workload": "BCC 32^3 AB diblock, N=100, f=0.25, chi N=20, ds=1", "cases": out}
 
 
+def design_leg(L, n_inst=32, seed=20261204):
+    """Batching plus differentiation: ABA triblock (N=100, blocks 20/60/20)
+    + A homopolymer (length 50, phi_h ~ U[0.05, 0.2]), chi N ~ U[15, 25],
+    ds = 0.5, contour slabs checkpointed every 14 steps.
+    (1) Gradient cost on 96^3 grids: the gradient of sum_r (phi_A - t)^2 in
+        omega(r), f_A and phi_h for n_inst instances on seeded fields, one
+        backward pass per instance against one grouped backward
+        (adjoint.Group).
+    (2) The descent of pscf_amd.design toward seeded target periods
+        d_t ~ U[12.5, 15.5] for the same instances as lamellae (1D cells, 96
+        points): |sigma(d_t)| per iteration."""
+    import torch
+    from pscf_amd import adjoint, design, scf
+    from pscf_amd.system import System
+    rng = np.random.default_rng(seed)
+    blends = [design.Blend(f_A=0.4, phi_h=float(rng.uniform(0.05, 0.2)),
+                           chi_N=float(rng.uniform(15.0, 25.0))) for _ in range(n_inst)]
+    n = 96
+    basis = scf.Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
+    G0 = np.eye(3) * (2.0 * np.pi / 24.0)
+    systems = []
+    for b in blends:
+        s = b.system(24.0, n, 0.5)
+        s = System(kuhn=s.kuhn, chains=s.chains, phi_chain=s.phi_chain, chi=s.chi, dim=3,
+                   crystal_system="cubic", cell_param=[24.0], ngrid=[n, n, n], chain_step=0.5,
+                   group_name="P 1")
+        S = scf.SCFSystem(s, basis, G0, lib=L)
+        S.set_checkpoint(14)
+        systems.append(S)
+    oms = [torch.tensor(0.1 * rng.standard_normal((2, n, n, n)), device="cuda") for _ in blends]
+    tgt = torch.tensor(0.5 + 0.1 * rng.standard_normal((n, n, n)), device="cuda")
+
+    def leaves(b):
+        fA = torch.tensor(b.f_A, dtype=torch.float64, requires_grad=True)
+        ph = torch.tensor(b.phi_h, dtype=torch.float64, requires_grad=True)
+        Lb = torch.stack([50.0 * fA, 100.0 * (1.0 - fA), 50.0 * fA,
+                          torch.tensor(50.0, dtype=torch.float64)])
+        return fA, ph, Lb, torch.stack([1.0 - ph, ph])
+
+    cp = torch.tensor([24.0], dtype=torch.float64)
+    torch.cuda.synchronize()
+    t_fwd = t_bwd = 0.0
+    loop_grads = []
+    for i, (b, S) in enumerate(zip(blends, systems)):
+        fA, ph, Lb, phis = leaves(b)
+        om = oms[i].clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        rho, Q = adjoint.density(S, om, cp, Lb, phis)
+        loss = ((rho[0] - tgt) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_bwd += t2 - t1
+        loop_grads.append((fA.grad.item(), ph.grad.item()))
+    G = adjoint.Group(systems)
+    res = {}
+    for rep in range(2):       # the first grouped pass builds and captures the launch set
+        lv = [leaves(b) for b in blends]
+        om = [o.clone().requires_grad_(True) for o in oms]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = G.density(om, [cp] * n_inst, [x[2] for x in lv], [x[3] for x in lv])
+        loss = sum(((rho[0] - tgt) ** 2).sum() for rho, _ in outs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = {"forward_s": t1 - t0, "backward_s": t2 - t1}
+        same = all(lv[i][0].grad.item() == loop_grads[i][0] and lv[i][1].grad.item() == loop_grads[i][1]
+                   for i in range(n_inst))
+    del G, outs, loss, om
+    systems.clear()
+    torch.cuda.empty_cache()
+    grad_cost = {"instances": n_inst, "grid": [n, n, n], "checkpoint_stride": 14,
+                 "forward_s_total": t_fwd, "backward_s_loop": t_bwd,
+                 "backward_s_grouped": res["backward_s"],
+                 "grouped_speedup": t_bwd / res["backward_s"],
+                 "backward_over_forward_grouped": res["backward_s"] / t_fwd,
+                 "grouped_equals_loop": bool(same)}
+    # (2) descent toward target periods, lamellae on 96 points
+    tables = design.lamellar_tables(n)
+    target = [float(x) for x in rng.uniform(12.5, 15.5, n_inst)]
+    om0 = np.zeros((2, n // 2 + 1))
+    t0 = time.perf_counter()
+    starts = []
+    for b in blends:
+        o = om0.copy()
+        o[0, 1] = 0.25 * b.chi_N / 100.0 * 4.0
+        o[1, 1] = -o[0, 1]
+        starts.append((o, o))
+    _, hist = design.fit_spacing(blends, target, starts, tables, n, 0.5, iters=4, lib=L)
+    t_design = time.perf_counter() - t0
+    sig = [float(np.abs(h["sigma"]).max()) for h in hist]
+    return {"metric": "batched adjoint gradients with checkpointing (ABA + A blends)",
+            "gradient_cost_96cubed": grad_cost,
+            "descent": {"instances": n_inst, "grid": [n], "iterations": len(hist) - 1,
+                        "max_abs_sigma_per_iteration": sig, "wall_s": t_design,
+                        "final_f_A_range": [float(hist[-1]["f_A"].min()), float(hist[-1]["f_A"].max())],
+                        "final_phi_h_range": [float(hist[-1]["phi_h"].min()),
+                                              float(hist[-1]["phi_h"].max())]}}
+
+
 def traffic_probe():
     import torch
     from pscf_amd import iterate, scf
@@ -349,6 +455,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient-cost leg")
     ap.add_argument("--no-jacobian", action="store_true", help="skip the Newton Jacobian leg")
+    ap.add_argument("--no-design", action="store_true",
+                    help="skip the batched-gradient / inverse-design leg")
     ap.add_argument("--task-scaling", action="store_true",
                     help="only print contour-step cost versus sweeps per launch (BCC 64^3)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -421,6 +529,8 @@ def main():
         extra["adjoint"] = adjoint_leg(L, g, r.omega, r.cell_param[0], elapsed / args.steps)
     if not args.no_jacobian:
         extra["jacobian"] = jacobian_leg(L)
+    if not args.no_design:
+        extra["design"] = design_leg(L)
     if not args.no_batch:
         extra["batch"] = batch_leg(L)
 

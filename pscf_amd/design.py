@@ -1,0 +1,181 @@
+"""Inverse design with adjoint gradients: the block fraction f_A and the
+homopolymer fraction phi_h of ABA triblock + A homopolymer blends are driven,
+per instance, toward a target lamellar period.
+
+At a converged SCFT state the free energy per monomer F* is stationary in
+the fields, so its total derivative in any parameter theta of the chains
+(block lengths, volume fractions) or of the cell is the partial derivative
+at fixed omega of the ideal-chain part
+    H = sum_c phi_c (ln phi_c - ln Q_c - 1) / N_c
+(the interaction and Lagrange-multiplier terms drop out because
+sum_alpha rho_alpha = 1 along the solutions; for the cell this is the
+reference's scf_stress, src/scf_mod.fp.f90:451-550).  One backward pass of
+density() (adjoint.Group, all instances' adjoint sweeps in one launch set,
+contour slabs checkpointed) gives sigma = dF*/dd and dF*/dtheta at once.
+
+A period d_t is an equilibrium spacing when sigma(d_t) = 0.  Per instance the
+loss L = sigma(d_t)^2 has gradient 2 sigma dsigma/dtheta, and
+dsigma/dtheta = d^2 F* / dd dtheta is the d-derivative of dF*/dtheta: the
+central difference of the adjoint theta-gradients at converged states at
+d_t (1 +- h).  Each iteration solves all instances at both cells (one
+batched Anderson solve, warm-started), runs one grouped backward pass over
+the 2n states, and takes a gradient step with the Gauss-Newton length
+(1 / (2 |dsigma/dtheta|^2)), limited to `max_step` per parameter.
+"""
+import copy
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import adjoint, batch
+from .system import System
+
+
+def lamellar_tables(n):
+    """Star tables of a 1D lamellar cell (group -1) on n grid points, in the
+    layout of basis_mod (src/basis_mod.f90:60-84): waves 0, +-1, ...,
+    +-(n/2 - 1), n/2; one star per |k|, coefficient 1/sqrt(count)."""
+    if n % 2:
+        raise ValueError("n must be even")
+    h = n // 2
+    wave = [0]
+    for k in range(1, h):
+        wave += [k, -k]
+    wave.append(h)
+    wave = np.array(wave, dtype=np.int32).reshape(-1, 1)
+    count = np.array([1] + [2] * (h - 1) + [1])
+    star_of_wave = np.concatenate([[1]] + [[k + 1, k + 1] for k in range(1, h)] + [[h + 1]])
+    coeff = np.array([1.0 / np.sqrt(c) for c in count for _ in range(c)], dtype=np.complex128)
+    return {"wave": wave, "coeff": coeff, "star_of_wave": star_of_wave.astype(np.int32),
+            "star_invert": np.zeros(h + 1, dtype=np.int32),
+            "wave_of_star": np.arange(h + 1, dtype=np.int32).reshape(-1, 1),
+            "star_count": count.astype(np.int32)}
+
+
+@dataclass
+class Blend:
+    """ABA triblock (length N, blocks f N/2, (1 - f) N, f N/2) + A homopolymer
+    (length N_h, volume fraction phi_h), chi N = chi_N, b_A = b_B = 1."""
+    f_A: float
+    phi_h: float
+    chi_N: float
+    N: float = 100.0
+    N_h: float = 50.0
+
+    def system(self, period, n_grid, ds):
+        chi = self.chi_N / self.N
+        f = self.f_A
+        return System(kuhn=[1.0, 1.0],
+                      chains=[[(1, 0.5 * f * self.N), (2, (1.0 - f) * self.N), (1, 0.5 * f * self.N)],
+                              [(1, self.N_h)]],
+                      phi_chain=[1.0 - self.phi_h, self.phi_h],
+                      chi=np.array([[0.0, chi], [chi, 0.0]]), dim=1, crystal_system="lamellar",
+                      cell_param=[float(period)], ngrid=[n_grid], chain_step=ds, group_name="-1")
+
+    def theta_jacobian(self):
+        """d(block lengths)/d(f_A, phi_h) and d(phi_chain)/d(f_A, phi_h)."""
+        dL = np.array([[0.5 * self.N, 0.0], [-self.N, 0.0], [0.5 * self.N, 0.0], [0.0, 0.0]])
+        dphi = np.array([[0.0, -1.0], [0.0, 1.0]])
+        return dL, dphi
+
+
+def ideal_free_energy(Q, lengths, phi, nblocks):
+    """H = sum_c phi_c (ln phi_c - ln Q_c - 1) / N_c, N_c = sum of its block lengths."""
+    H = torch.zeros((), dtype=torch.float64)
+    k = 0
+    for c, nb in enumerate(nblocks):
+        Nc = lengths[k:k + nb].sum()
+        k += nb
+        H = H + phi[c] * (torch.log(phi[c]) - torch.log(Q[c].cpu()) - 1.0) / Nc
+    return H
+
+
+def stress_and_gradient(blends, target, omega_start, tables, n_grid, ds, h=1e-3, stride=14,
+                        error_max=1e-9, lib=None):
+    """sigma(d_t), dsigma/d(f_A, phi_h) per instance (module docstring), and
+    the converged fields at d_t (1 -+ h) for the next warm start."""
+    from ._lib import lib as _default_lib
+    lib = lib or _default_lib()
+    n = len(blends)
+    cells = [(t * (1.0 - h), t * (1.0 + h)) for t in target]
+    solver = batch.BatchSolver(n_hist=20, domain=False, error_max=error_max, max_itr=2000, lib=lib)
+    for i, b in enumerate(blends):
+        for side in (0, 1):
+            s = b.system(cells[i][side], n_grid, ds)
+            solver.add(batch.make_system(s, tables, lib=lib), omega_start[i][side])
+    solver.solve()
+    res = [solver.result(k) for k in range(2 * n)]
+    bad = [k for k, r in enumerate(res) if not r.converge]
+    if bad:
+        raise RuntimeError("SCFT solves did not converge: %s" % bad)
+    # grouped backward pass over the 2n converged states, contour checkpointed
+    states = []
+    for i, b in enumerate(blends):
+        for side in (0, 1):
+            s = b.system(cells[i][side], n_grid, ds)
+            S = batch.make_system(s, tables, lib=lib)
+            if stride:
+                S.set_checkpoint(stride)
+            states.append((S, s))
+    G = adjoint.Group([S for S, _ in states])
+    dev = lib.tensor_device
+    om_grids, cps, Ls, phs = [], [], [], []
+    for k, (S, s) in enumerate(states):
+        om = torch.tensor(res[k].omega, dtype=torch.float64, device=dev)
+        om_grids.append(adjoint.stars_to_grid(S, om).detach())
+        cps.append(torch.tensor(np.array(s.cell_param, dtype=np.float64), requires_grad=True))
+        Ls.append(adjoint.block_lengths(s).requires_grad_(True))
+        phs.append(torch.tensor(np.array(s.phi_chain, dtype=np.float64), requires_grad=True))
+    outs = G.density(om_grids, cps, Ls, phs)
+    H = sum(ideal_free_energy(Q, Ls[k], phs[k], [len(c) for c in states[k][1].chains])
+            for k, (_, Q) in enumerate(outs))
+    H.backward()
+    sigma = np.zeros(n)
+    dsig = np.zeros((n, 2))
+    for i, b in enumerate(blends):
+        dL, dphi = b.theta_jacobian()
+        sig_side, ht = [], []
+        for side in (0, 1):
+            k = 2 * i + side
+            sig_side.append(float(cps[k].grad[0]))
+            ht.append(Ls[k].grad.numpy() @ dL + phs[k].grad.numpy() @ dphi)
+        sigma[i] = 0.5 * (sig_side[0] + sig_side[1])
+        dsig[i] = (ht[1] - ht[0]) / (cells[i][1] - cells[i][0])
+    omega_next = [(res[2 * i].omega, res[2 * i + 1].omega) for i in range(n)]
+    return sigma, dsig, omega_next
+
+
+def omega_gap(omega_pair):
+    """Relative difference of the converged fields at d_t (1 -+ h): O(h) on
+    one branch of solutions; O(1) when the two solves landed on different
+    branches (e.g. one or two periods per cell), where the difference
+    quotient of the gradients means nothing."""
+    a, b = (np.asarray(x)[:, 1:] for x in omega_pair)
+    return float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+
+
+def fit_spacing(blends, target, omega_start, tables, n_grid, ds, iters=5, max_step=0.05,
+                bounds=((0.1, 0.9), (0.01, 0.5)), **kw):
+    """Gradient descent of L_i = sigma_i(d_t,i)^2 in (f_A, phi_h) of every
+    instance.  Returns the final blends and a history of (sigma, f_A, phi_h)
+    per iteration."""
+    blends = [copy.deepcopy(b) for b in blends]
+    omega = [tuple(o) for o in omega_start]
+    hist = []
+    for it in range(iters + 1):
+        sigma, dsig, omega = stress_and_gradient(blends, target, omega, tables, n_grid, ds, **kw)
+        hist.append({"sigma": sigma.copy(), "f_A": np.array([b.f_A for b in blends]),
+                     "phi_h": np.array([b.phi_h for b in blends]),
+                     "grad": 2.0 * sigma[:, None] * dsig,
+                     "omega_gap": np.array([omega_gap(o) for o in omega])})
+        if it == iters:
+            break
+        for i, b in enumerate(blends):
+            g = 2.0 * sigma[i] * dsig[i]
+            nrm = float(dsig[i] @ dsig[i])
+            step = -g / (2.0 * nrm) if nrm > 0 else np.zeros(2)
+            step = np.clip(step, -max_step, max_step)
+            b.f_A = float(np.clip(b.f_A + step[0], *bounds[0]))
+            b.phi_h = float(np.clip(b.phi_h + step[1], *bounds[1]))
+    return blends, hist
