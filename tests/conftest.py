@@ -48,9 +48,12 @@ def get_lib(kind):
         return _libs[kind]
     from pscf_amd._lib import Lib, DEFAULT_LIB
     if kind == "emu":
-        if _emu_stale():
-            import subprocess
-            subprocess.run([os.path.join(EMU_DIR, "build_emu.sh")], check=True)
+        import fcntl
+        with open(os.path.join(EMU_DIR, ".build.lock"), "w") as lk:   # one builder at a time
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            if _emu_stale():
+                import subprocess
+                subprocess.run([os.path.join(EMU_DIR, "build_emu.sh")], check=True)
         L = Lib(EMU_LIB)
         L.tensor_device = "cpu"      # emulated device memory is host memory
     else:
