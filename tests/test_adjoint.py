@@ -68,8 +68,8 @@ H = 1e-3
 @pytest.mark.parametrize("name,step", [("bcc8", None), ("hex24", 0.025), ("blend_nosolv", 0.09)])
 def test_gradients_match_central_fd(backend, name, step):
     if backend == "emu" and name == "blend_nosolv":
-        pytest.skip("two-chain FD on the GPU only (a minute emulated); the emulated two-chain "
-                    "adjoint is pinned against the GPU's by the checkpoint and one-workgroup tests")
+        pytest.skip("two-chain FD on the GPU only (a minute emulated); on the emulator the "
+                    "two-chain adjoint runs in the checkpoint and one-workgroup tests")
     S, s, g, omg, dev = _make(name, backend, step)
     rng = np.random.default_rng(11)
     tgt = torch.tensor(rng.normal(0.3, 0.05, omg.shape[1:]), device=dev)
