@@ -27,6 +27,16 @@ from .iterate import unit_cell
 
 N_HIST = 3    # src/sweep_mod.f90:97
 
+# The reference writes the step-schedule constants as default-real
+# (single-precision) literals (src/pscf.fp.f90:590-697): the steps are
+# 0.1f and 0.9f times the step unit, so after the first two points s is
+# 0.99999997765..., not 1, and every swept parameter carries that offset.
+# The same values here keep the path's parameters equal to the reference's.
+_F01 = float(np.float32(0.1))
+_F09 = float(np.float32(0.9))
+_F0001 = float(np.float32(0.001))
+_BACKTRACK_MIN = float(np.float32(np.float32(1.0 / 16.0) + np.float32(0.001)))
+
 
 @dataclass
 class Increments:
@@ -93,9 +103,9 @@ class _State:
     def next_step(self):
         """src/pscf.fp.f90:582-593."""
         if self.i == 0:
-            return 0.1 * self.step_unit
+            return _F01 * self.step_unit
         if self.i == 1:
-            return 0.9 * self.step_unit if self.converge else self.step_unit - self.s
+            return _F09 * self.step_unit if self.converge else self.step_unit - self.s
         return self.step_unit
 
     def increment(self, step):
@@ -196,9 +206,9 @@ def run(paths, domain=False, n_hist=20, error_max=1e-8, max_itr=500, lib=None, l
                                               error=r.error))
                 if log:
                     log("path %d s=%.4f itr=%d f=%.10f" % (states.index(st), st.s, r.itr, r.f_helmholtz))
-                if st.s + 0.001 >= st.p.s_max:
+                if st.s + _F0001 >= st.p.s_max:
                     st.p.status = "done"
-            elif st.step_unit > (1.0 / 16.0) + 0.001:
+            elif st.step_unit > _BACKTRACK_MIN:
                 st.s = st.s - st.step
                 st.increment(-st.step)
                 st.step_unit = 0.5 * st.step_unit
