@@ -41,11 +41,16 @@ def _need(path):
         pytest.skip("%s not built (make -C oracle; make -C pscf_amd/fortran)" % path)
 
 
-def _write_case(tmp_path, max_itr, algo="AM"):
+def _write_case(tmp_path, max_itr, algo="AM", ensemble=0):
     # start: a field within ~5% of the lamellar solution (tests/golden/lam32_start.npz),
     # so that both programs take the same, non-chaotic Anderson path
     g = load_golden("lam32")
     sysm = cases()["lam32"][0]
+    if ensemble:      # the same melt at the known answer's chemical potential
+        import copy
+        sysm = copy.deepcopy(sysm)
+        sysm.ensemble = 1
+        sysm.mu_chain = [KNOWN["mu_chain"]]
     B = scf.Basis.from_tables(g)
     om = load_golden("lam32_start")["omega"]
     fieldio.write_field(str(tmp_path / "in.omega"), om, sysm, B, g["star_count"])
@@ -137,3 +142,37 @@ def test_dropin_converges_to_known_answer_gpu(tmp_path, algo):
         assert abs(a[k] / v - 1) < 2e-10, ("reference", k, a[k], v)
         assert abs(b[k] / v - 1) < 2e-10, ("gpu drop-in", k, b[k], v)
         assert abs(b[k] / a[k] - 1) < 1e-9, (k, a[k], b[k])
+
+
+@pytest.mark.gpu
+def test_dropin_newton_grand_ensemble_gpu(tmp_path):
+    """iterate_NR in the grand ensemble (mu_chain at the known answer's value).
+    The Jacobian's perturbed density() calls have no qout, so the reference
+    builds them with the phi_chain it was given (src/scf_mod.fp.f90:202-218):
+    the drop-in must too.  The two iteration logs agree to the FD rounding
+    amplification, and both converge to the canonical known answer."""
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("gpu")
+    _write_case(tmp_path, max_itr=100, algo="NR", ensemble=1)
+    ref_dir, gpu_dir = tmp_path / "ref", tmp_path / "gpu"
+    for d in (ref_dir, gpu_dir):
+        d.mkdir()
+        for f in ("param", "in.omega"):
+            (d / f).write_text((tmp_path / f).read_text())
+        (d / "out").mkdir()
+    ref = _log(_run(REF, str(ref_dir)))
+    mine = _log(_run(GPU_BIN, str(gpu_dir)))
+    assert len(ref) >= 4
+    for (i, e, f), (j, e2, f2) in zip(ref[:4], mine[:4]):
+        # the first Newton step from residual 1.07 carries the FD rounding
+        # amplification (~1e-4 in the Jacobian) into 1e-5 / 1e-6 differences;
+        # a Jacobian built from updated phi_chain would be O(1) off
+        assert i == j
+        assert abs(e2 / e - 1) < 1e-4, (i, e, e2)
+        assert abs(f2 / f - 1) < 2e-6, (i, f, f2)
+    a = _summary(str(ref_dir / "out" / "out"))
+    b = _summary(str(gpu_dir / "out" / "out"))
+    for k in ("f_Helmholtz", "pressure"):
+        assert abs(b[k] / a[k] - 1) < 1e-9, (k, a[k], b[k])
+        assert abs(b[k] / KNOWN[k] - 1) < 1e-8, (k, b[k], KNOWN[k])
