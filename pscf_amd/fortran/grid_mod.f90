@@ -1,0 +1,233 @@
+!-----------------------------------------------------------------------
+! grid_mod -- drop-in replacement of src/grid_mod.f90 with the same public
+! variables and procedures.
+!
+! The driver recomputes |G|^2 on the half grid after every cell update
+! (make_ksq, src/iterate_mod.fp.f90:1455): 27 minimum-image trials for each
+! of the Mk wavevectors, tens of milliseconds per call at 64^3 on the host,
+! which a GPU iteration would otherwise wait for.  Once the device system
+! exists, make_ksq hands the cell to it and copies back the |G|^2 table that
+! the library computes on the device with the same first-found minimum-image
+! rule (pscf_system_set_cell / pscf_system_get_ksq; the table is bit-exact
+! against this module's host loop).  Before that (star construction at
+! start-up) and in the other procedures, the host code below keeps the
+! reference's arithmetic: the trial order +N, 0, -N per dimension, the
+! 1.0E-8 tie margin and group_mod's dot products (src/grid_mod.f90:141-413).
+!-----------------------------------------------------------------------
+module grid_mod
+   use const_mod
+   implicit none
+
+   private
+   public :: ngrid          ! grid dimensions (N1, N2, N3)
+   public :: rho_grid       ! rho on the r-grid (r, N_monomer)
+   public :: omega_grid     ! omega on the r-grid (r, N_monomer)
+   public :: ksq_grid       ! |G|^2 on the half k-grid
+   public :: input_grid
+   public :: allocate_grid
+   public :: deallocate_grid
+   public :: make_ksq
+   public :: max_Gabs
+   public :: G_to_fft
+   public :: G_to_bz
+   public :: Greal_to_bz
+   public :: norm
+   public :: ksq_hook, ksq_hook_t
+
+   integer                  :: ngrid(3)
+   real(long), ALLOCATABLE  :: rho_grid(:,:,:,:)
+   real(long), ALLOCATABLE  :: omega_grid(:,:,:,:)
+   real(long), ALLOCATABLE  :: ksq_grid(:,:,:)
+
+   ! |G|^2 table from the device: set by the drop-in's device state once the
+   ! system exists; done = .false. leaves the work to the host loop
+   abstract interface
+      subroutine ksq_hook_t(G_basis, ksq, done)
+         import :: long
+         real(long), intent(IN)  :: G_basis(:,:)
+         real(long), intent(OUT) :: ksq(:,:,:)
+         logical,    intent(OUT) :: done
+      end subroutine ksq_hook_t
+   end interface
+   procedure(ksq_hook_t), pointer :: ksq_hook => null()
+
+contains
+
+   !--------------------------------------------------------------------
+   ! input_grid                                   src/grid_mod.f90:84-93
+   !--------------------------------------------------------------------
+   subroutine input_grid
+   use io_mod, only : input
+   call input(ngrid(:), dim, f='A')
+   if (dim < 3) ngrid(dim+1:3) = 1
+   end subroutine input_grid
+
+   !--------------------------------------------------------------------
+   ! allocate_grid(N_monomer)                    src/grid_mod.f90:106-127
+   !--------------------------------------------------------------------
+   subroutine allocate_grid(N_monomer)
+   integer, intent(IN) :: N_monomer
+   integer :: info
+   allocate(rho_grid(0:ngrid(1)-1, 0:ngrid(2)-1, 0:ngrid(3)-1, N_monomer), stat=info)
+   if (info /= 0) stop "rho_grid allocation error!"
+   allocate(omega_grid(0:ngrid(1)-1, 0:ngrid(2)-1, 0:ngrid(3)-1, N_monomer), stat=info)
+   if (info /= 0) stop "rho_grid allocation error!"
+   allocate(ksq_grid(0:ngrid(1)/2, 0:ngrid(2)-1, 0:ngrid(3)-1), stat=info)
+   if (info /= 0) stop "rho_grid allocation error!"
+   end subroutine allocate_grid
+
+   !--------------------------------------------------------------------
+   ! make_ksq(G_basis)                           src/grid_mod.f90:141-160
+   ! ksq_grid(k) = |G_to_bz(k) . G_basis|^2 on the half grid
+   !--------------------------------------------------------------------
+   subroutine make_ksq(G_basis)
+   real(long) :: G_basis(:,:)
+   integer    :: k(3), kx, ky, kz
+   logical    :: done
+   if (associated(ksq_hook)) then
+      call ksq_hook(G_basis, ksq_grid, done)
+      if (done) return
+   end if
+   do kx = 0, ngrid(1)/2
+      do ky = 0, ngrid(2) - 1
+         do kz = 0, ngrid(3) - 1
+            k = (/ kx, ky, kz /)
+            ksq_grid(kx, ky, kz) = norm(G_to_bz(k), G_basis)
+         end do
+      end do
+   end do
+   end subroutine make_ksq
+
+   !--------------------------------------------------------------------
+   ! deallocate_grid                             src/grid_mod.f90:172-186
+   !--------------------------------------------------------------------
+   subroutine deallocate_grid
+   integer :: info
+   deallocate(ksq_grid, stat=info)
+   if (info /= 0) stop "rho_grid deallocation error!"
+   deallocate(rho_grid, stat=info)
+   if (info /= 0) stop "rho_grid deallocation error!"
+   deallocate(omega_grid, stat=info)
+   if (info /= 0) stop "rho_grid deallocation error!"
+   end subroutine deallocate_grid
+
+   !--------------------------------------------------------------------
+   ! G_to_fft(G): FFT-grid index of wavevector G   src/grid_mod.f90:199-209
+   !--------------------------------------------------------------------
+   function G_to_fft(G)
+   integer             :: G_to_fft(3)
+   integer, intent(IN) :: G(:)
+   integer :: d
+   G_to_fft = 0
+   do d = 1, dim
+      G_to_fft(d) = modulo(G(d), ngrid(d))
+   end do
+   end function G_to_fft
+
+   !--------------------------------------------------------------------
+   ! G_to_bz(G): the image G + (n1 N1, n2 N2, n3 N3), n in {1, 0, -1}, of
+   ! smallest |.|^2 under unit_cell_mod's G_basis; images are tried with n
+   ! descending in each dimension, and a later one replaces the current
+   ! choice only if shorter by more than 1.0E-8   src/grid_mod.f90:225-283
+   !--------------------------------------------------------------------
+   function G_to_bz(G)
+   use unit_cell_mod, only : G_basis
+   integer             :: G_to_bz(3)
+   integer, intent(IN) :: G(:)
+   integer    :: best(3), t(3), n(3), m1, m2, m3, n_hi(3)
+   real(long) :: best_sq, sq
+   best = 0
+   t = 0
+   best_sq = 1.0E+10
+   n_hi = 0
+   n_hi(1:dim) = 1
+   do m1 = n_hi(1), -n_hi(1), -1
+      do m2 = n_hi(2), -n_hi(2), -1
+         do m3 = n_hi(3), -n_hi(3), -1
+            n = (/ m1, m2, m3 /)
+            t(1:dim) = G(1:dim) + n(1:dim) * ngrid(1:dim)
+            sq = norm(t, G_basis)
+            if (sq < best_sq - 1.0E-8) then
+               best(1:dim) = t(1:dim)
+               best_sq = sq
+            end if
+         end do
+      end do
+   end do
+   G_to_bz = best
+   end function G_to_bz
+
+   !--------------------------------------------------------------------
+   ! Greal_to_bz(Greal): G_to_bz for a real-valued wavevector
+   !                                             src/grid_mod.f90:286-353
+   !--------------------------------------------------------------------
+   function Greal_to_bz(Greal)
+   use unit_cell_mod, only : G_basis
+   use group_mod,     only : operator(.dot.)
+   real(long)             :: Greal_to_bz(3)
+   real(long), intent(IN) :: Greal(:)
+   real(long) :: best(3), t(3), v(3), best_sq, sq
+   integer    :: n(3), m1, m2, m3, n_hi(3), d
+   best = 0.0_long
+   t = 0.0_long
+   best_sq = 1.0E+10
+   n_hi = 0
+   n_hi(1:dim) = 1
+   do m1 = n_hi(1), -n_hi(1), -1
+      do m2 = n_hi(2), -n_hi(2), -1
+         do m3 = n_hi(3), -n_hi(3), -1
+            n = (/ m1, m2, m3 /)
+            do d = 1, dim
+               t(d) = Greal(d) + n(d) * ngrid(d)
+            end do
+            v = 0.0_long
+            do d = 1, dim
+               v(d) = t(:) .dot. G_basis(:, d)
+            end do
+            sq = (v .dot. v)
+            if (sq < best_sq - 1.0E-8) then
+               best(1:dim) = t(1:dim)
+               best_sq = sq
+            end if
+         end do
+      end do
+   end do
+   Greal_to_bz = best
+   end function Greal_to_bz
+
+   !--------------------------------------------------------------------
+   ! max_Gabs(G_basis): largest |G| over the first Brillouin-zone images of
+   ! the grid                                    src/grid_mod.f90:367-390
+   !--------------------------------------------------------------------
+   function max_Gabs(G_basis)
+   real(long)             :: max_Gabs
+   real(long), intent(IN) :: G_basis(:,:)
+   real(long) :: longest, sq
+   integer    :: k(3), kx, ky, kz
+   longest = 1.0E-15
+   do kz = 0, ngrid(3) - 1
+      do ky = 0, ngrid(2) - 1
+         do kx = 0, ngrid(1) - 1
+            k = (/ kx, ky, kz /)
+            sq = norm(G_to_bz(k), G_basis)
+            if (sq > longest + 1.0E-10) longest = sq
+         end do
+      end do
+   end do
+   max_Gabs = sqrt(longest)
+   end function max_Gabs
+
+   !--------------------------------------------------------------------
+   ! norm(G, G_basis) = |G . G_basis|^2          src/grid_mod.f90:402-413
+   !--------------------------------------------------------------------
+   function norm(G, G_basis)
+   use group_mod, only : operator(.dot.)
+   real(long)             :: norm
+   integer, intent(IN)    :: G(:)
+   real(long), intent(IN) :: G_basis(:,:)
+   real(long) :: Gvec(3)
+   Gvec = G .dot. G_basis
+   norm = Gvec .dot. Gvec
+   end function norm
+
+end module grid_mod

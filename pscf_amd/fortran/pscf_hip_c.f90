@@ -13,6 +13,7 @@ module pscf_hip_c
    public :: c_pscf_chain_discretize
    public :: c_pscf_system_create, c_pscf_system_destroy, c_pscf_system_set_basis
    public :: c_pscf_system_set_cell, c_pscf_system_set_chemistry, c_pscf_system_set_chi
+   public :: c_pscf_system_get_ksq
    public :: c_pscf_system_startup, c_pscf_density, c_pscf_density_many, c_pscf_scf_stress
    public :: c_pscf_mu_phi_chain, c_pscf_mu_phi_solvent, c_pscf_free_energy
    public :: c_pscf_free_energy_fh, c_pscf_set_omega_uniform, c_pscf_divide_energy_terms
@@ -104,6 +105,11 @@ module pscf_hip_c
          import :: c_int, c_int64_t, c_double
          integer(c_int64_t), value :: sys
          real(c_double), intent(IN) :: G_basis(*)
+      end function
+      integer(c_int) function c_pscf_system_get_ksq(sys, ksq) bind(C, name='pscf_system_get_ksq')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(OUT) :: ksq(*)
       end function
       integer(c_int) function c_pscf_system_set_chemistry(sys, n_monomer, kuhn, n_chain, n_block, &
             n_blk_max, block_monomer, block_length, n_solvent, solvent_monomer, solvent_size, &

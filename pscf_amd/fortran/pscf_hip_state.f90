@@ -5,12 +5,14 @@
 ! here one libpscf_hip system mirrors them: the star tables are uploaded once
 ! after make_basis, the cell whenever unit_cell_mod's G_basis changes (the
 ! driver calls make_unit_cell/make_ksq itself: src/iterate_mod.fp.f90:1453-1459),
-! and the chemistry at density_startup and before every density().
+! and the chemistry at density_startup and before every density().  Once the
+! system exists, grid_mod's make_ksq takes |G|^2 from it (ksq_on_device).
 !-----------------------------------------------------------------------
 module pscf_hip_state
    use, intrinsic :: iso_c_binding
    use const_mod
    use pscf_hip_c
+   use grid_mod, only : ksq_hook
    implicit none
    private
    public :: hsys, ensure_system, sync_cell, sync_chemistry, sync_chi
@@ -40,7 +42,24 @@ contains
       have_cell = .false.
       call sync_cell()
       call sync_chemistry()
+      ksq_hook => ksq_on_device
    end subroutine ensure_system
+
+   ! grid_mod's make_ksq on the device: upload the cell, copy |G|^2 back
+   subroutine ksq_on_device(G_basis, ksq, done)
+      real(long), intent(IN)  :: G_basis(:,:)
+      real(long), intent(OUT) :: ksq(:,:,:)
+      logical,    intent(OUT) :: done
+      real(c_double) :: g(dim, dim)
+      done = .false.
+      if (hsys == 0) return
+      g = G_basis(1:dim, 1:dim)
+      call hip_check(c_pscf_system_set_cell(hsys, g), 'make_ksq')
+      last_G(1:dim,1:dim) = G_basis(1:dim,1:dim)
+      have_cell = .true.
+      call hip_check(c_pscf_system_get_ksq(hsys, ksq), 'make_ksq')
+      done = .true.
+   end subroutine ksq_on_device
 
    ! upload G_basis when it differs from the last upload
    subroutine sync_cell()
