@@ -134,27 +134,39 @@ contains
    use unit_cell_mod, only : G_basis
    integer             :: G_to_bz(3)
    integer, intent(IN) :: G(:)
-   integer    :: best(3), t(3), n(3), m1, m2, m3, n_hi(3)
-   real(long) :: best_sq, sq
+   integer    :: best(3), t(3), m1, m2, m3, h2, h3
+   real(long) :: best_sq
    best = 0
    t = 0
    best_sq = 1.0E+10
-   n_hi = 0
-   n_hi(1:dim) = 1
-   do m1 = n_hi(1), -n_hi(1), -1
-      do m2 = n_hi(2), -n_hi(2), -1
-         do m3 = n_hi(3), -n_hi(3), -1
-            n = (/ m1, m2, m3 /)
-            t(1:dim) = G(1:dim) + n(1:dim) * ngrid(1:dim)
-            sq = norm(t, G_basis)
-            if (sq < best_sq - 1.0E-8) then
-               best(1:dim) = t(1:dim)
-               best_sq = sq
-            end if
+   h2 = 0
+   h3 = 0
+   if (dim >= 2) h2 = 1
+   if (dim >= 3) h3 = 1
+   do m1 = 1, -1, -1
+      t(1) = G(1) + m1 * ngrid(1)
+      do m2 = h2, -h2, -1
+         if (dim >= 2) t(2) = G(2) + m2 * ngrid(2)
+         do m3 = h3, -h3, -1
+            if (dim >= 3) t(3) = G(3) + m3 * ngrid(3)
+            call keep_shorter
          end do
       end do
    end do
    G_to_bz = best
+
+   contains
+
+      ! the trial t replaces the current choice if shorter by > 1.0E-8
+      subroutine keep_shorter
+      real(long) :: sq
+      sq = norm(t, G_basis)
+      if (sq < best_sq - 1.0E-8) then
+         best(1:dim) = t(1:dim)
+         best_sq = sq
+      end if
+      end subroutine keep_shorter
+
    end function G_to_bz
 
    !--------------------------------------------------------------------
