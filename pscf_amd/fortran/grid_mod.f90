@@ -129,16 +129,22 @@ contains
    ! smallest |.|^2 under unit_cell_mod's G_basis; images are tried with n
    ! descending in each dimension, and a later one replaces the current
    ! choice only if shorter by more than 1.0E-8   src/grid_mod.f90:225-283
+   ! Star construction calls this for every wave and symmetry operation
+   ! (millions of times at 64^3), so |.|^2 is evaluated inline, with the
+   ! operations of norm() in norm()'s order: the same values, without two
+   ! array-valued calls per image.
    !--------------------------------------------------------------------
    function G_to_bz(G)
    use unit_cell_mod, only : G_basis
    integer             :: G_to_bz(3)
    integer, intent(IN) :: G(:)
-   integer    :: best(3), t(3), m1, m2, m3, h2, h3
-   real(long) :: best_sq
+   integer    :: best(3), t(3), m1, m2, m3, h2, h3, i, j
+   real(long) :: best_sq, sq, gv, b(3,3)
    best = 0
    t = 0
    best_sq = 1.0E+10
+   b = 0.0_long
+   b(1:dim, 1:dim) = G_basis(1:dim, 1:dim)
    h2 = 0
    h3 = 0
    if (dim >= 2) h2 = 1
@@ -149,24 +155,22 @@ contains
          if (dim >= 2) t(2) = G(2) + m2 * ngrid(2)
          do m3 = h3, -h3, -1
             if (dim >= 3) t(3) = G(3) + m3 * ngrid(3)
-            call keep_shorter
+            sq = 0.0_long
+            do j = 1, dim
+               gv = 0.0_long
+               do i = 1, dim
+                  gv = gv + t(i) * b(i, j)
+               end do
+               sq = sq + gv * gv
+            end do
+            if (sq < best_sq - 1.0E-8) then
+               best(1:dim) = t(1:dim)
+               best_sq = sq
+            end if
          end do
       end do
    end do
    G_to_bz = best
-
-   contains
-
-      ! the trial t replaces the current choice if shorter by > 1.0E-8
-      subroutine keep_shorter
-      real(long) :: sq
-      sq = norm(t, G_basis)
-      if (sq < best_sq - 1.0E-8) then
-         best(1:dim) = t(1:dim)
-         best_sq = sq
-      end if
-      end subroutine keep_shorter
-
    end function G_to_bz
 
    !--------------------------------------------------------------------
