@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdlib>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -79,6 +82,29 @@ struct DevBuf {
   void upload(const T* h, size_t count, hipStream_t s) {
     alloc(count);
     HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// Page-locked host staging: copies between it and the device are truly
+// asynchronous, so a stream's work can be queued for many systems before
+// one synchronisation (a copy from pageable memory waits for the stream).
+template <class T>
+struct HostPin {
+  T* p = nullptr;
+  size_t n = 0;
+  HostPin() = default;
+  HostPin(const HostPin&) = delete;
+  HostPin& operator=(const HostPin&) = delete;
+  ~HostPin() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    HIPCHK(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), 0));
+    n = count;
   }
 };
 
@@ -388,7 +414,10 @@ struct System {
   DevBuf<double> q1;       // m per task
   DevBuf<StepTask> tasks;
   std::vector<StepTask> h_tasks;
-  std::vector<double> h_Q, h_Qs, h_chem, h_dots;
+  std::vector<double> h_Q, h_Qs, h_chem;
+  // staging of the per-call host inputs and outputs (see HostPin)
+  HostPin<double> pin_in, pin_chem, pin_q, pin_rho, pin_dgsq, pin_dots;
+  double* rho_dst = nullptr;     // where density_finish copies the rho stars
   bool have_density = false;
   // ensemble of the chain prefactors of the last density(): the reference
   // updates phi_chain = Q e^mu first only when qout is present
@@ -844,7 +873,10 @@ void density_pre(System& S, const double* omega, const double* phi_c, const doub
   S.chain_ens = (S.ensemble == 1 && chain_update) ? 1 : 0;
   if (!grid_in) {
     const size_t nst = (size_t)S.nmon * S.nstar;
-    S.omega_in.upload(omega, nst, s);
+    S.pin_in.alloc(nst);
+    std::memcpy(S.pin_in.p, omega, sizeof(double) * nst);
+    S.omega_in.alloc(nst);
+    HIPCHK(hipMemcpyAsync(S.omega_in.p, S.pin_in.p, sizeof(double) * nst, hipMemcpyHostToDevice, s));
     basis_to_grid(S, S.omega_in.p, S.nmon, S.omega_grid.p, s);
   }
   for (int c = 0; c < S.nchain; ++c) {
@@ -865,7 +897,9 @@ void density_pre(System& S, const double* omega, const double* phi_c, const doub
     S.h_chem[2 * (S.nchain + i)] = phi_s ? phi_s[i] : 0.0;
     S.h_chem[2 * (S.nchain + i) + 1] = mu_s ? mu_s[i] : 0.0;
   }
-  HIPCHK(hipMemcpyAsync(S.chem_dev.p, S.h_chem.data(), sizeof(double) * S.h_chem.size(),
+  S.pin_chem.alloc(S.h_chem.size());
+  std::memcpy(S.pin_chem.p, S.h_chem.data(), sizeof(double) * S.h_chem.size());
+  HIPCHK(hipMemcpyAsync(S.chem_dev.p, S.pin_chem.p, sizeof(double) * S.h_chem.size(),
                         hipMemcpyHostToDevice, s));
 }
 
@@ -954,16 +988,24 @@ void density_post(System& S, double* rho, hipStream_t s) {
     grid_to_basis(S, S.rho_grid.p, S.nmon, 1.0 / (double)g.m, S.rho_out.p, s);
   }
   HIPCHK(hipGetLastError());
-  S.h_Q.assign(S.nchain + S.nsolv, 0.0);
-  HIPCHK(hipMemcpyAsync(S.h_Q.data(), S.qdev.p, sizeof(double) * (S.nchain + S.nsolv),
+  S.pin_q.alloc(S.nchain + S.nsolv);
+  HIPCHK(hipMemcpyAsync(S.pin_q.p, S.qdev.p, sizeof(double) * (S.nchain + S.nsolv),
                         hipMemcpyDeviceToHost, s));
-  if (rho) HIPCHK(hipMemcpyAsync(rho, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
+  S.rho_dst = rho;
+  if (rho) {
+    S.pin_rho.alloc(nst);
+    HIPCHK(hipMemcpyAsync(S.pin_rho.p, S.rho_out.p, sizeof(double) * nst, hipMemcpyDeviceToHost, s));
+  }
 }
 
 // after the stream is synchronised: outputs and the grand-ensemble update (:202-209)
 void density_finish(System& S, double* qout, double* qsolv, double* phi_c, double* mu_c,
                     double* phi_s, double* mu_s) {
   if (S.profile) collect_profile(S, S.max_ns);
+  S.h_Q.assign(S.pin_q.p, S.pin_q.p + (S.nchain + S.nsolv));
+  if (S.rho_dst)
+    std::memcpy(S.rho_dst, S.pin_rho.p, sizeof(double) * (size_t)S.nmon * S.nstar);
+  S.rho_dst = nullptr;
   for (int c = 0; c < S.nchain; ++c)
     if (qout) qout[c] = S.h_Q[c];
   for (int i = 0; i < S.nsolv; ++i)
@@ -1725,12 +1767,16 @@ void adjoint_group(AdjGroup& G, const double* const* lam, const double* const* d
 // ---- scf_stress (src/scf_mod.fp.f90:451-550)
 
 // per contour point p of every chain: sum_star qr_basis * dGsq * qf_basis
-// (:505-528) on the device, copied to the host (S.h_dots)
+// (:505-528) on the device, copied to the host (S.pin_dots)
 void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
   REQUIRE(S.have_density, "scf_stress requires a previous density()");
   REQUIRE(S.ck_stride == 0, "scf_stress needs every contour point (checkpointed storage is on)");
   const Grid& g = S.g;
-  S.dgsq.upload(dgsq, (size_t)S.nstar * ncp, s);
+  S.pin_dgsq.alloc((size_t)S.nstar * ncp);
+  std::memcpy(S.pin_dgsq.p, dgsq, sizeof(double) * S.nstar * ncp);
+  S.dgsq.alloc((size_t)S.nstar * ncp);
+  HIPCHK(hipMemcpyAsync(S.dgsq.p, S.pin_dgsq.p, sizeof(double) * S.nstar * ncp,
+                        hipMemcpyHostToDevice, s));
   const int chunk = 32;
   S.sbasis.alloc((size_t)2 * chunk * S.nstar);
   S.skbuf.alloc((size_t)chunk * g.mk);
@@ -1756,8 +1802,8 @@ void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
     off += (size_t)ch.ns + 1;
   }
   HIPCHK(hipGetLastError());
-  S.h_dots.resize(npts * ncp);
-  HIPCHK(hipMemcpyAsync(S.h_dots.data(), S.sdots.p, sizeof(double) * npts * ncp,
+  S.pin_dots.alloc(npts * ncp);
+  HIPCHK(hipMemcpyAsync(S.pin_dots.p, S.sdots.p, sizeof(double) * npts * ncp,
                         hipMemcpyDeviceToHost, s));
 }
 
@@ -1779,7 +1825,7 @@ void stress_finish(System& S, int ncp, const double* phi_c, const double* mu_c, 
         double ds = ds0;
         if (i != ibgn && i != iend) ds = (i % 2 == 0) ? 4.0 * ds : 2.0 * ds;
         for (int beta = 0; beta < ncp; ++beta) {
-          double inc = S.h_dots[(off + (size_t)(i - 1)) * ncp + beta];
+          double inc = S.pin_dots.p[(off + (size_t)(i - 1)) * ncp + beta];
           inc = inc * (bk * bk) * ds / normal;
           dQ[beta] = dQ[beta] - inc;
         }
@@ -1969,6 +2015,41 @@ void mu_phi(BatchInst& B) {
   }
 }
 
+// Run f(0..n-1) on up to OMP_NUM_THREADS (default: 16, at most the hardware
+// threads) host threads; the first exception is rethrown on the caller.
+template <class F>
+void parallel_for(size_t n, F&& f) {
+  size_t T = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 1) T = (size_t)v;
+  }
+  T = std::min(T, n);
+  if (T <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::exception_ptr err;
+  std::mutex m;
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < T; ++t)
+    th.emplace_back([&] {
+      for (;;) {
+        const size_t i = next++;
+        if (i >= n) break;
+        try {
+          f(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> lk(m);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+
 void batch_solve(Batch& Bt) {
   REQUIRE(!Bt.inst.empty(), "empty batch");
   if (Bt.groups.empty()) build_groups(Bt);
@@ -2009,8 +2090,14 @@ void batch_solve(Batch& Bt) {
     if (!any) break;
     ++Bt.iterations;
     batch_density(Bt, Bt.domain != 0);
-    for (auto& B : Bt.inst) {
-      if (!B.active) continue;
+    // the instances' host work (thermodynamics, Anderson step, new cell) is
+    // independent: it runs on host threads, the device cell updates after
+    std::vector<BatchInst*> act;
+    for (auto& B : Bt.inst)
+      if (B.active) act.push_back(&B);
+    std::vector<std::vector<double>> Gnew(act.size());
+    parallel_for(act.size(), [&](size_t k) {
+      BatchInst& B = *act[k];
       System& S = *B.S;
       ++B.itr;
       mu_phi(B);
@@ -2023,19 +2110,25 @@ void batch_solve(Batch& Bt) {
       if (st != 0) {
         B.status = st;
         B.active = false;
-        continue;
+        return;
       }
       if (Bt.domain) {
         const int dim = S.g.dim;
-        std::vector<double> G((size_t)dim * dim);
-        if (pscf_unit_cell(dim, B.cs.c_str(), B.ncp, B.cell.data(), nullptr, G.data(), nullptr))
+        Gnew[k].assign((size_t)dim * dim, 0.0);
+        if (pscf_unit_cell(dim, B.cs.c_str(), B.ncp, B.cell.data(), nullptr, Gnew[k].data(), nullptr))
           throw Fail{std::string("unit cell: ") + pscf_am_last_error()};
-        for (int i = 0; i < 9; ++i) S.gb[i] = 0.0;
-        for (int i = 0; i < dim; ++i)
-          for (int jj = 0; jj < dim; ++jj) S.gb[i * 3 + jj] = G[i + dim * jj];
-        apply_cell(S);
         dgsq_of(B);
       }
+    });
+    for (size_t k = 0; k < act.size(); ++k) {
+      BatchInst& B = *act[k];
+      if (!B.active || !Bt.domain) continue;
+      System& S = *B.S;
+      const int dim = S.g.dim;
+      for (int i = 0; i < 9; ++i) S.gb[i] = 0.0;
+      for (int i = 0; i < dim; ++i)
+        for (int jj = 0; jj < dim; ++jj) S.gb[i * 3 + jj] = Gnew[k][i + dim * jj];
+      apply_cell(S);
     }
     for (auto& B : Bt.inst) B.S->st.sync();   // cell updates run on each system's stream
   }
