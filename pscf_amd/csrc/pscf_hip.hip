@@ -188,9 +188,18 @@ ZSet make_z() {
   return s;
 }
 
-// Grid sizes with compiled kernels: cubic/square grids of these sizes, and
-// 1D/2D grids whose unused dimensions are 1.
+// Grid edges with compiled kernels.  A grid N1 x N2 x N3 runs when its XY
+// plane (N1, N2) and its XZ pair (N1, N3) both have kernels: edges from this
+// list, equal (cubic, square; 1D/2D grids have unused dimensions 1) or
+// within a factor 2 of each other (PSCF_GRID_PAIRS: tetragonal, hexagonal,
+// orthorhombic and rectangular cells on anisotropic grids).
 #define PSCF_GRID_SIZES(X) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(96)
+#define PSCF_GRID_PAIRS(X) X(8, 12) X(8, 16) X(12, 8) X(12, 16) X(12, 20) X(12, 24) X(16, 8) \
+   X(16, 12) X(16, 20) X(16, 24) X(16, 32) X(20, 12) X(20, 16) X(20, 24) X(20, 32) X(20, 40) \
+   X(24, 12) X(24, 16) X(24, 20) X(24, 32) X(24, 40) X(24, 48) X(32, 16) X(32, 20) X(32, 24) \
+   X(32, 40) X(32, 48) X(32, 64) X(40, 20) X(40, 24) X(40, 32) X(40, 48) X(40, 64) X(48, 24) \
+   X(48, 32) X(48, 40) X(48, 64) X(48, 96) X(64, 32) X(64, 40) X(64, 48) X(64, 96) X(96, 48) \
+   X(96, 64)
 
 const std::map<std::pair<int, int>, PlaneSet>& plane_table() {
   static const std::map<std::pair<int, int>, PlaneSet> t = [] {
@@ -200,6 +209,9 @@ const std::map<std::pair<int, int>, PlaneSet>& plane_table() {
   m[{n, 1}] = make_plane<n, 1>();
     PSCF_GRID_SIZES(PSCF_PL)
 #undef PSCF_PL
+#define PSCF_PL2(a, b) m[{a, b}] = make_plane<a, b>();
+    PSCF_GRID_PAIRS(PSCF_PL2)
+#undef PSCF_PL2
     return m;
   }();
   return t;
@@ -212,6 +224,9 @@ const std::map<std::pair<int, int>, ZSet>& z_table() {
   m[{n, 1}] = make_z<n, 1>();
     PSCF_GRID_SIZES(PSCF_Z)
 #undef PSCF_Z
+#define PSCF_Z2(a, b) m[{a, b}] = make_z<a, b>();
+    PSCF_GRID_PAIRS(PSCF_Z2)
+#undef PSCF_Z2
     return m;
   }();
   return t;

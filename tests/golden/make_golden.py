@@ -78,6 +78,27 @@ def cases():
     C["bcc64"] = (diblock(0.25, 100.0, 20.0, dim=3, crystal_system="cubic", cell_param=[18.0],
                           ngrid=[64, 64, 64], chain_step=1.0, group_name="I m -3 m"),
                   20261205, dict(grids=False, step=False, nstar_seed=11))
+    # anisotropic grids: every edge pair differs somewhere
+    C["tet12x16"] = (System(kuhn=[1.0, 1.1], chains=[[(1, 0.35), (2, 0.65)]], phi_chain=[1.0],
+                            chi=_chi2(16.0), dim=3, crystal_system="tetragonal",
+                            cell_param=[1.7, 2.3], ngrid=[12, 12, 16], chain_step=0.05,
+                            group_name="P 4/m m m"),
+                     20261206, dict(grids=True, nstar_seed=10))
+    C["orth16x12x24"] = (System(kuhn=[1.0, 1.0], chains=[[(1, 0.4), (2, 0.6)]], phi_chain=[1.0],
+                                chi=_chi2(15.0), dim=3, crystal_system="orthorhombic",
+                                cell_param=[2.0, 1.6, 2.9], ngrid=[16, 12, 24], chain_step=0.05,
+                                group_name="P m m m"),
+                         20261207, dict(grids=True, nstar_seed=10))
+    C["rect24x16"] = (System(kuhn=[1.0, 1.0], chains=[[(1, 0.3), (2, 0.7)]], phi_chain=[1.0],
+                             chi=_chi2(18.0), dim=2, crystal_system="rectangular",
+                             cell_param=[2.1, 1.5], ngrid=[24, 16], chain_step=0.02,
+                             group_name="p 2 m m"),
+                      20261208, dict(grids=True, nstar_seed=8))
+    # the O70 (Fddd) network, N=100 diblock, on 32 x 48 x 64
+    C["fddd32x48x64"] = (diblock(0.42, 100.0, 14.0, dim=3, crystal_system="orthorhombic",
+                                 cell_param=[9.5, 16.0, 32.0], ngrid=[32, 48, 64],
+                                 chain_step=1.0, group_name="F d d d : 2"),
+                         20261209, dict(grids=False, step=True, nstar_seed=11))
     return C
 
 

@@ -24,8 +24,9 @@ from make_golden import cases  # noqa: E402
 import pscf_oracle as O  # noqa: E402
 from pscf_amd import scf  # noqa: E402
 
-SMALL = ["bcc8", "lam32", "hex24", "p213_12", "blend12"]
-LARGE = ["bcc32", "gyr64", "lam_tet32", "hex48", "bcc64"]
+SMALL = ["bcc8", "lam32", "hex24", "p213_12", "blend12",
+         "tet12x16", "orth16x12x24", "rect24x16"]      # the last three: anisotropic grids
+LARGE = ["bcc32", "gyr64", "lam_tet32", "hex48", "bcc64", "fddd32x48x64"]
 
 
 def _cases(backend):
@@ -199,22 +200,23 @@ def test_contour_slabs_match_oracle(backend):
         assert rel(S.q(0, -1, s), ch.qr[s]) < 1e-12, s
 
 
-def _grid_density_vs_oracle(backend, n, steps_ds):
-    """density() on an n^3 grid (every compiled plane/z kernel plan of that
-    length) from a seeded random omega grid, against the oracle's
-    chain_density: AB diblock N=10, f=0.3, cubic cell a = n/4."""
+def _grid_density_vs_oracle(backend, ng, steps_ds):
+    """density() on an N1 x N2 x N3 grid (every compiled plane/z kernel plan
+    of those lengths) from a seeded random omega grid, against the oracle's
+    chain_density: AB diblock N=10, f=0.3, orthorhombic cell of edges N_i/4."""
     import ctypes
     import torch
-    from pscf_amd import batch
     from pscf_amd.system import diblock
     lib = get_lib(backend)
-    s = diblock(0.3, 10.0, 15.0, dim=3, crystal_system="cubic", cell_param=[n / 4.0],
-                ngrid=[n, n, n], chain_step=steps_ds, group_name="P 1")
-    G = np.eye(3) * (2.0 * np.pi / (n / 4.0))
+    n1, n2, n3 = ng
+    s = diblock(0.3, 10.0, 15.0, dim=3, crystal_system="orthorhombic",
+                cell_param=[n1 / 4.0, n2 / 4.0, n3 / 4.0], ngrid=list(ng), chain_step=steps_ds,
+                group_name="P 1")
+    G = np.diag([2.0 * np.pi / (n / 4.0) for n in ng])
     basis = scf.Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
     S = scf.SCFSystem(s, basis, G, lib=lib)
-    rng = np.random.default_rng(n)
-    om = 1.5 + 0.3 * rng.standard_normal((2, n, n, n))
+    rng = np.random.default_rng(n1 * 10000 + n2 * 100 + n3)
+    om = 1.5 + 0.3 * rng.standard_normal((2, n3, n2, n1))
     dev = lib.tensor_device
     omt = torch.tensor(om, device=dev)
     rho = torch.empty_like(omt)
@@ -228,8 +230,8 @@ def _grid_density_vs_oracle(backend, n, steps_ds):
              dptr(S.mu_chain), dptr(S.phi_solvent), dptr(S.mu_solvent))
     chem = O.Chemistry(s.kuhn, s.chains, s.phi_chain, s.chi)
     bgn, bds = O.chain_grid([L for _, L in s.chains[0]], steps_ds)
-    ch = O.chain_density(0, O.ChainState(bgn, bds), chem, O.make_ksq([n, n, n], G), om,
-                         O.Propagator([n, n, n]))
+    ch = O.chain_density(0, O.ChainState(bgn, bds), chem, O.make_ksq(list(ng), G), om,
+                         O.Propagator(list(ng)))
     ref = np.zeros_like(om)
     for ib, (mon, _) in enumerate(s.chains[0]):
         ref[mon - 1] += s.phi_chain[0] * ch.rho[ib]
@@ -237,17 +239,30 @@ def _grid_density_vs_oracle(backend, n, steps_ds):
     assert rel(rho.cpu().numpy(), ref) < 1e-12
 
 
-@pytest.mark.parametrize("n", [12, 20])
-def test_grid_density_small_sizes(backend, n):
-    _grid_density_vs_oracle(backend, n, 1.0)
+@pytest.mark.parametrize("ng", [(12, 12, 12), (20, 20, 20), (20, 12, 24)])
+def test_grid_density_small_sizes(backend, ng):
+    _grid_density_vs_oracle(backend, ng, 1.0)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [40, 96])
-def test_grid_density_large_sizes_gpu(n):
-    """40^3 (radix-5 factor) and 96^3 (the plan with one k_step_a workgroup
-    per plane and radix-3 factors)."""
-    _grid_density_vs_oracle("gpu", n, 2.5)
+@pytest.mark.parametrize("ng", [(40, 40, 40), (96, 96, 96), (64, 48, 96), (96, 64, 48)])
+def test_grid_density_large_sizes_gpu(ng):
+    """40^3 (radix-5 factor), 96^3 (the plan with one k_step_a workgroup per
+    plane and radix-3 factors) and two anisotropic grids whose XY planes and
+    XZ pairs are all of unequal edges."""
+    _grid_density_vs_oracle("gpu", ng, 2.5)
+
+
+def test_unsupported_grid_shape_is_rejected(backend):
+    """Edge pairs without compiled kernels (here 8 and 32, a factor 4 apart)
+    fail when the system is created, with the grid in the message."""
+    from pscf_amd.system import diblock
+    lib = get_lib(backend)
+    s = diblock(0.3, 10.0, 15.0, dim=3, crystal_system="tetragonal", cell_param=[2.0, 8.0],
+                ngrid=[8, 8, 32], chain_step=1.0, group_name="P 1")
+    basis = scf.Basis(np.zeros((1, 3)), np.ones(1), np.ones(1), np.zeros(1), np.zeros((1, 3)))
+    with pytest.raises(RuntimeError, match="8x8x32"):
+        scf.SCFSystem(s, basis, np.diag([np.pi, np.pi, np.pi / 4]), lib=lib)
 
 
 def test_one_workgroup_step_kernel_emu(monkeypatch):
