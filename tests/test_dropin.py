@@ -274,3 +274,60 @@ def test_dropin_bcc32_trajectory_matches_reference_gpu(tmp_path):
         assert i == j
         assert abs(e2 / e - 1) < 1e-7, (i, e, e2)
         assert abs(f2 / f - 1) < 1e-9, (i, f, f2)
+
+
+def _star_case_dirs(tmp_path, name, max_itr, domain):
+    """The reference program and the drop-in on the same param file and seeded
+    star field of a golden case (Anderson, N_hist 20)."""
+    g = load_golden(name)
+    sysm = cases()[name][0]
+    B = scf.Basis.from_tables(g)
+    om = np.array(g["omega"], dtype=np.float64)
+    fieldio.write_field(str(tmp_path / "in.omega"), om, sysm, B, g["star_count"])
+    text = sysm.to_param() + (
+        "ITERATE\ninput_filename\n'in.omega'\noutput_prefix\n'out/'\nmax_itr\n%d\n"
+        "error_max\n1.0E-10\ndomain\n%s\nitr_algo\n'AM'\nN_hist\n20\n\nFINISH\n"
+        % (max_itr, "T" if domain else "F"))
+    dirs = tmp_path / "ref", tmp_path / "gpu"
+    for d in dirs:
+        d.mkdir()
+        (d / "param").write_text(text)
+        (d / "in.omega").write_text((tmp_path / "in.omega").read_text())
+        (d / "out").mkdir()
+    return dirs
+
+
+def _same_log(ref, mine, n, tol_err):
+    assert len(ref) >= n and len(mine) == len(ref)
+    for (i, e, f), (j, e2, f2) in zip(ref, mine):
+        assert i == j
+        assert abs(e2 / e - 1) < tol_err, (i, e, e2)
+        assert abs(f2 / f - 1) < 1e-9, (i, f, f2)
+
+
+@pytest.mark.parametrize("name", ["tet12x16", "orth16x12x24"])
+def test_dropin_anisotropic_grid_matches_reference_cpu(tmp_path, name):
+    """Anisotropic grids through the drop-in (tetragonal 12x12x16, two cell
+    parameters; orthorhombic 16x12x24, three), with cell relaxation: the
+    first Anderson iterations' log against the reference program's."""
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("emu")
+    ref_dir, gpu_dir = _star_case_dirs(tmp_path, name, 4, True)
+    ref = _log(_run(REF, str(ref_dir)))
+    mine = _log(_run(GPU_BIN, str(gpu_dir), env=dict(os.environ, LD_PRELOAD=EMU_LIB)))
+    _same_log(ref, mine, 4, 1e-7)
+
+
+@pytest.mark.gpu
+def test_dropin_fddd_trajectory_matches_reference_gpu(tmp_path):
+    """The Fddd network (N = 100, f = 0.42, chi N = 14) on a 32 x 48 x 64
+    grid, three cell parameters relaxed: eight Anderson iterations against
+    the reference program's log."""
+    _need(REF)
+    _need(GPU_BIN)
+    get_lib("gpu")
+    ref_dir, gpu_dir = _star_case_dirs(tmp_path, "fddd32x48x64", 8, True)
+    ref = _log(_run(REF, str(ref_dir)))
+    mine = _log(_run(GPU_BIN, str(gpu_dir)))
+    _same_log(ref, mine, 8, 1e-7)
