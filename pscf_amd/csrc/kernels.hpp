@@ -561,12 +561,15 @@ __global__ void __launch_bounds__(256) k_adj_kgrad(const StepTask* __restrict__ 
 // Standalone transform passes (batched over blockIdx.y arrays).
 // ----------------------------------------------------------------------------
 
-// real planes -> x,y forward transformed half planes; in[b*istride + ...]
+// real planes -> x,y forward transformed half planes; in[b*istride + ...].
+// With `cols`, only the ncolp half-plane positions cols[c] are stored, at
+// out[b*ostride + z*ncolp + c].
 template <int N1, int N2>
 __global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_r2c(const double* __restrict__ in, long istride,
                                                   double2* __restrict__ out, long ostride,
                                                   const double2* __restrict__ tw1g,
-                                                  const double2* __restrict__ tw2g) {
+                                                  const double2* __restrict__ tw2g,
+                                                  const int* __restrict__ cols, int ncolp) {
   constexpr int NT = plane_nt(N1, N2);
   using PL = Plane<N1, N2>;
   __shared__ double2 U[PL::USIZE];
@@ -583,6 +586,11 @@ __global__ void __launch_bounds__(plane_nt(N1, N2)) k_plane_r2c(const double* __
   }
   __syncthreads();
   plane_forward<N1, N2, NT>(P, H, tw1, tw2, tid);
+  if (cols) {
+    double2* dst = out + blockIdx.y * ostride + (long)z * ncolp;
+    for (int c = tid; c < ncolp; c += NT) dst[c] = H[cols[c]];
+    return;
+  }
   double2* dst = out + blockIdx.y * ostride + (long)z * PL::HSIZE;
   for (int i = tid; i < PL::HSIZE; i += NT) dst[i] = H[i];
 }

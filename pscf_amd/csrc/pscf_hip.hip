@@ -134,7 +134,8 @@ const double2* twiddles(int n) {
 // ------------------------------------------------------- kernel dispatch
 using StepPlaneFn = void (*)(const StepTask*, int, int, const double2*, const double2*);
 using StepZFn = void (*)(const StepTask*, int, int, int, const double2*);
-using R2CFn = void (*)(const double*, long, double2*, long, const double2*, const double2*);
+using R2CFn = void (*)(const double*, long, double2*, long, const double2*, const double2*,
+                       const int*, int);
 using C2RFn = void (*)(const double2*, long, double*, long, double, const double2*, const double2*);
 using ZfftFn = void (*)(double2*, long, int, const double2*);
 using C2CFn = void (*)(double2*, long, const double2*, const double2*);
@@ -279,9 +280,22 @@ struct Grid {
 
   // real (nb arrays, stride m) -> half spectra (stride mk)
   void r2c(const double* in, double2* out, int nb, hipStream_t s) const {
-    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(pl->nt), 0, s, in, m, out, mk, tw1, tw2);
+    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(pl->nt), 0, s, in, m, out, mk, tw1, tw2,
+                       (const int*)nullptr, 0);
     if (n3 > 1)
       hipLaunchKernelGGL(zs->half_f, dim3(n2, nb), dim3(zs->nt_half), 0, s, out, mk, n2, tw3);
+  }
+  // the same transform kept on ncolp columns of every XY half plane (cols:
+  // their positions; ncolp a multiple of n1h), out[b][z][c] with stride
+  // n3*ncolp: the z pass runs on ncolp/n1h rows of n1h columns
+  void r2c_cols(const double* in, double2* out, int nb, const int* cols, int ncolp,
+                hipStream_t s) const {
+    const long st = (long)n3 * ncolp;
+    hipLaunchKernelGGL(pl->r2c, dim3(n3, nb), dim3(pl->nt), 0, s, in, m, out, st, tw1, tw2, cols,
+                       ncolp);
+    if (n3 > 1)
+      hipLaunchKernelGGL(zs->half_f, dim3(ncolp / n1h, nb), dim3(zs->nt_half), 0, s, out, st,
+                         ncolp / n1h, tw3);
   }
   // half spectra -> real, in place on `in` for the z pass
   void c2r(double2* in, double* out, int nb, double scale, hipStream_t s) const {
@@ -408,6 +422,13 @@ struct System {
   DevBuf<int> star_of_wave, star_invert, wtab, kwave, sconj;
   DevBuf<double2> coeff, scoef;
   DevBuf<long> sidx;
+  // stress transforms keep only the half-spectrum columns (kx, ky) holding a
+  // star's representative wave: ncolp columns (a multiple of N1/2+1, padded
+  // with column 0), their positions in an XY half plane, and per star the
+  // index of its representative in that compact (z, column) layout
+  int ncolp = 0;
+  DevBuf<int> scols;
+  DevBuf<long> sidx_c;
   std::vector<int> h_wave, h_wave_of_star, h_star_of_wave, h_star_invert;
   std::vector<double2> h_coeff;
   int gmin[3] = {0, 0, 0}, gext[3] = {1, 1, 1};
@@ -640,6 +661,22 @@ void build_basis_maps(System& S) {
     sidx[s] = kf[0] + (long)S.g.n1h * (kf[1] + (long)S.g.n2 * kf[2]);
   }
   S.sidx.upload(sidx.data(), sidx.size(), S.st.s);
+  const long plane = (long)S.g.n1h * S.g.n2;
+  std::vector<int> cols;
+  for (int s = 0; s < S.nstar; ++s) cols.push_back((int)(sidx[s] % plane));
+  std::sort(cols.begin(), cols.end());
+  cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+  const int ncol = (int)cols.size();
+  S.ncolp = (ncol + S.g.n1h - 1) / S.g.n1h * S.g.n1h;
+  std::vector<long> sidx_c(S.nstar);
+  for (int s = 0; s < S.nstar; ++s) {
+    const int c = (int)(std::lower_bound(cols.begin(), cols.end(), (int)(sidx[s] % plane)) -
+                        cols.begin());
+    sidx_c[s] = c + (long)S.ncolp * (sidx[s] / plane);
+  }
+  cols.resize(S.ncolp, 0);
+  S.scols.upload(cols.data(), cols.size(), S.st.s);
+  S.sidx_c.upload(sidx_c.data(), sidx_c.size(), S.st.s);
   S.sconj.upload(sconj.data(), sconj.size(), S.st.s);
   S.scoef.upload(scoef.data(), scoef.size(), S.st.s);
   S.st.sync();
@@ -1793,8 +1830,9 @@ void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
   HIPCHK(hipMemcpyAsync(S.dgsq.p, S.pin_dgsq.p, sizeof(double) * S.nstar * ncp,
                         hipMemcpyHostToDevice, s));
   const int chunk = 32;
+  const long mkc = (long)g.n3 * S.ncolp;      // one point's compact spectrum
   S.sbasis.alloc((size_t)2 * chunk * S.nstar);
-  S.skbuf.alloc((size_t)chunk * g.mk);
+  S.skbuf.alloc((size_t)chunk * mkc);
   size_t npts = 0;
   for (int sp = 0; sp < S.nchain; ++sp) npts += (size_t)S.chains[sp]->ns + 1;
   S.sdots.alloc(npts * ncp);
@@ -1805,9 +1843,9 @@ void stress_launch(System& S, int ncp, const double* dgsq, hipStream_t s) {
       const int np = std::min(chunk, ch.ns + 1 - p0);
       for (int d = 0; d < 2; ++d) {
         const double* src = (d == 0 ? ch.qf.p : ch.qr.p) + (size_t)p0 * g.m;
-        g.r2c(src, S.skbuf.p, np, s);
+        g.r2c_cols(src, S.skbuf.p, np, S.scols.p, S.ncolp, s);
         hipLaunchKernelGGL(k_kgrid_to_basis, dim3(grid_blocks((long)S.nstar * np)), dim3(NTA), 0, s,
-                           S.skbuf.p, g.mk, np, S.nstar, S.sidx.p, S.sconj.p, S.scoef.p,
+                           S.skbuf.p, mkc, np, S.nstar, S.sidx_c.p, S.sconj.p, S.scoef.p,
                            S.star_invert.p, 1.0, S.sbasis.p + (size_t)d * chunk * S.nstar, 0);
       }
       hipLaunchKernelGGL(k_stress_dots, dim3(np), dim3(NTA), 0, s, S.sbasis.p,
