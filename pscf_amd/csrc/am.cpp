@@ -1,6 +1,7 @@
 // Host-side iteration support (FP64, CPU): unit-cell geometry, dGsq and the
 // Anderson-mixing update of the reference's iterate_AM.  These are O(N_star)
 // per iteration and stay on the host, as in the reference.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -135,9 +136,11 @@ void unit_cell(int dim, const std::string& cs, int ncp, const double* cp, double
 struct AM {
   int nstar = 0, nmon = 0, ncp = 0, nhist = 0, domain = 0, itr = 0;
   int n1 = 0;                                  // N_star - 1 coefficients per monomer
-  std::vector<double> dev, om_hist, dev_cp, cp_hist;   // [h][a][k], [h][c]
-  double* d(int h, int a) { return &dev[((size_t)h * nmon + a) * n1]; }
-  double* o(int h, int a) { return &om_hist[((size_t)h * nmon + a) * n1]; }
+  std::vector<double> dev, om_hist, dev_cp, cp_hist;   // [slot][a][k], [h][c]
+  int h0 = 0;       // slot of history entry 0 of dev/om_hist (a ring of nhist + 1)
+  size_t slot(int h) const { return (size_t)((h0 + h) % (nhist + 1)); }
+  double* d(int h, int a) { return &dev[(slot(h) * nmon + a) * n1]; }
+  double* o(int h, int a) { return &om_hist[(slot(h) * nmon + a) * n1]; }
 };
 
 // LU with partial pivoting, then solve U x = ... for the n right-hand side
@@ -235,6 +238,7 @@ int pscf_am_start(int64_t handle, const double* omega, const double* cell_param)
   return guard([&] {
     AM& A = *reinterpret_cast<AM*>(handle);
     A.itr = 0;
+    A.h0 = 0;
     std::fill(A.dev.begin(), A.dev.end(), 0.0);
     std::fill(A.om_hist.begin(), A.om_hist.end(), 0.0);
     std::fill(A.dev_cp.begin(), A.dev_cp.end(), 0.0);
@@ -312,15 +316,36 @@ int pscf_am_iterate(int64_t handle, double* omega, const double* rho, const doub
     } else {
       const int np = itr_prev;
       std::vector<double> U((size_t)np * np, 0.0), V(np, 0.0);
+      // U(m,n) = sum_a sum_k (dc - dm)(dc - dn) and V(m) = sum_a sum_k (dc - dm) dc
+      // (:1265-1380).  One pass over k per monomer feeds an accumulator per
+      // (m, n >= m) and per m; each accumulator still sums over k in order, so
+      // the values are those of one loop per pair, at a fraction of the time
+      // (the pairs' chains run side by side instead of one after another).
+      std::vector<double> su((size_t)np * np), sv(np), t(np);
+      std::vector<const double*> dp(np + 1);
+      for (int a = 0; a < nm; ++a) {
+        for (int m = 0; m <= np; ++m) dp[m] = A.d(cur - m, a);
+        std::fill(su.begin(), su.end(), 0.0);
+        std::fill(sv.begin(), sv.end(), 0.0);
+        const double* dc = dp[0];
+        for (int k = 0; k < n1; ++k) {
+          const double dck = dc[k];
+          for (int m = 0; m < np; ++m) t[m] = dck - dp[m + 1][k];
+          for (int m = 0; m < np; ++m) {
+            const double tm = t[m];
+            double* row = su.data() + (size_t)m * np;
+            for (int n = m; n < np; ++n) row[n] += tm * t[n];
+            sv[m] += tm * dck;
+          }
+        }
+        for (int m = 0; m < np; ++m) {
+          for (int n = m; n < np; ++n) U[(size_t)m * np + n] += su[(size_t)m * np + n];
+          V[m] += sv[m];
+        }
+      }
       for (int m = 1; m <= np; ++m) {
         for (int n = m; n <= np; ++n) {
-          double u = 0.0;
-          for (int a = 0; a < nm; ++a) {
-            const double *dc = A.d(cur, a), *dm = A.d(cur - m, a), *dn = A.d(cur - n, a);
-            double s = 0.0;
-            for (int k = 0; k < n1; ++k) s += (dc[k] - dm[k]) * (dc[k] - dn[k]);
-            u += s;
-          }
+          double u = U[(size_t)(m - 1) * np + (n - 1)];
           if (A.domain) {
             double s = 0.0;
             for (int c = 0; c < ncp; ++c)
@@ -331,21 +356,13 @@ int pscf_am_iterate(int64_t handle, double* omega, const double* rho, const doub
           U[(size_t)(m - 1) * np + (n - 1)] = u;
           U[(size_t)(n - 1) * np + (m - 1)] = u;
         }
-        double v = 0.0;
-        for (int a = 0; a < nm; ++a) {
-          const double *dc = A.d(cur, a), *dm = A.d(cur - m, a);
-          double s = 0.0;
-          for (int k = 0; k < n1; ++k) s += (dc[k] - dm[k]) * dc[k];
-          v += s;
-        }
         if (A.domain) {
           double s = 0.0;
           for (int c = 0; c < ncp; ++c)
             s += (A.dev_cp[(size_t)cur * ncp + c] - A.dev_cp[(size_t)(cur - m) * ncp + c]) *
                  A.dev_cp[(size_t)cur * ncp + c];
-          v += s;
+          V[m - 1] += s;
         }
-        V[m - 1] = v;
       }
       std::vector<double> Cn = np == 1 ? std::vector<double>{V[0] / U[0]} : solve(U, V, np);
       for (int a = 0; a < nm; ++a) {
@@ -381,9 +398,9 @@ int pscf_am_iterate(int64_t handle, double* omega, const double* rho, const doub
       if (A.domain)
         for (int c = 0; c < ncp; ++c) A.cp_hist[(size_t)itr * ncp + c] = cell_param[c];
     } else {
-      const size_t blk = (size_t)nm * n1;
-      std::memmove(A.dev.data(), A.dev.data() + blk, sizeof(double) * blk * A.nhist);
-      std::memmove(A.om_hist.data(), A.om_hist.data() + blk, sizeof(double) * blk * A.nhist);
+      // drop entry 0: entries 1..nhist become 0..nhist-1 (the ring turns by
+      // one; entry nhist's old contents are overwritten before they are read)
+      A.h0 = (A.h0 + 1) % (A.nhist + 1);
       for (int a = 0; a < nm; ++a)
         for (int k = 0; k < n1; ++k) A.o(A.nhist, a)[k] = om(a, k);
       if (A.domain) {
