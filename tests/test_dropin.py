@@ -305,29 +305,29 @@ def _same_log(ref, mine, n, tol_err):
         assert abs(f2 / f - 1) < 1e-9, (i, f, f2)
 
 
-@pytest.mark.parametrize("name", ["tet12x16", "orth16x12x24"])
-def test_dropin_anisotropic_grid_matches_reference_cpu(tmp_path, name):
-    """Anisotropic grids through the drop-in (tetragonal 12x12x16, two cell
-    parameters; orthorhombic 16x12x24, three), with cell relaxation: the
-    first Anderson iterations' log against the reference program's."""
+def test_dropin_anisotropic_grid_matches_reference_cpu(tmp_path):
+    """An anisotropic grid through the drop-in (tetragonal 12x12x16, two cell
+    parameters), with cell relaxation: the first Anderson iterations' log
+    against the reference program's."""
     _need(REF)
     _need(GPU_BIN)
     get_lib("emu")
-    ref_dir, gpu_dir = _star_case_dirs(tmp_path, name, 4, True)
+    ref_dir, gpu_dir = _star_case_dirs(tmp_path, "tet12x16", 4, True)
     ref = _log(_run(REF, str(ref_dir)))
     mine = _log(_run(GPU_BIN, str(gpu_dir), env=dict(os.environ, LD_PRELOAD=EMU_LIB)))
     _same_log(ref, mine, 4, 1e-7)
 
 
 @pytest.mark.gpu
-def test_dropin_fddd_trajectory_matches_reference_gpu(tmp_path):
-    """The Fddd network (N = 100, f = 0.42, chi N = 14) on a 32 x 48 x 64
-    grid, three cell parameters relaxed: eight Anderson iterations against
-    the reference program's log."""
+@pytest.mark.parametrize("name", ["orth16x12x24", "fddd32x48x64"])
+def test_dropin_orthorhombic_trajectory_matches_reference_gpu(tmp_path, name):
+    """Orthorhombic cells, three parameters relaxed: P mmm on 16 x 12 x 24 and
+    the Fddd network (N = 100, f = 0.42, chi N = 14) on 32 x 48 x 64; eight
+    Anderson iterations against the reference program's log."""
     _need(REF)
     _need(GPU_BIN)
     get_lib("gpu")
-    ref_dir, gpu_dir = _star_case_dirs(tmp_path, "fddd32x48x64", 8, True)
+    ref_dir, gpu_dir = _star_case_dirs(tmp_path, name, 8, True)
     ref = _log(_run(REF, str(ref_dir)))
     mine = _log(_run(GPU_BIN, str(gpu_dir)))
     _same_log(ref, mine, 8, 1e-7)
